@@ -1,0 +1,460 @@
+"""ctypes binding of libgz_hip.so (include/gz_hip.h) with the reference's class and method names.
+
+`Sketch` mirrors include/sketch.h, `CCSketchAlg` mirrors include/cc_sketch_alg.h and
+`CCAlgConfiguration` mirrors include/cc_alg_configuration.h.  Errors surface as the reference's
+exceptions (OutOfSamplesException, UpdateLockedException).  There is no CPU fallback: if the
+library is missing or no GPU is usable, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgz_hip.so")
+
+GZ_OK, GZ_ERR_INVALID_ARG, GZ_ERR_OUT_OF_SAMPLES, GZ_ERR_UPDATE_LOCKED = 0, 1, 2, 3
+GZ_ERR_OOM, GZ_ERR_DEVICE, GZ_ERR_ADMISSION, GZ_ERR_IO = 4, 5, 6, 7
+GOOD, ZERO, FAIL = 0, 1, 2  # SampleResult, include/sketch.h:20-24
+CONNECTIVITY, KSPANNINGFORESTS = 0, 1  # QueryCode, include/cc_sketch_alg.h:60-63
+INSERT, DELETE = 0, 1
+
+
+class GZError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[gz status {code}] {msg}")
+        self.code = code
+
+
+class OutOfSamplesException(GZError):
+    """include/sketch.h:193-204"""
+
+
+class UpdateLockedException(GZError):
+    """include/cc_sketch_alg.h:25-29"""
+
+
+class AdmissionError(GZError):
+    """Serving: the tenant's footprint does not fit the memory budget."""
+
+
+class _AlgConfig(ctypes.Structure):
+    _fields_ = [
+        ("sketches_factor", ctypes.c_double),
+        ("batch_factor", ctypes.c_double),
+        ("gamma_bytes", ctypes.c_int),
+        ("eager_dsu", ctypes.c_int),
+        ("batched", ctypes.c_int),
+        ("tile", ctypes.c_uint32),
+        ("flush_threshold", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+
+_SIGS = {
+    "gz_last_error": (ctypes.c_char_p, []),
+    "gz_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "gz_calc_vector_length": (ctypes.c_uint64, [ctypes.c_uint32]),
+    "gz_calc_cc_samples": (_sz, [ctypes.c_uint32, ctypes.c_double]),
+    "gz_calc_bkt_per_col": (_sz, [ctypes.c_uint64]),
+    "gz_sketch_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, _sz, _sz, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "gz_sketch_destroy": (ctypes.c_int, [_vp]),
+    "gz_sketch_update": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "gz_sketch_update_batch": (ctypes.c_int, [_vp, _vp, _sz]),
+    "gz_sketch_sample": (ctypes.c_int, [_vp, _u64p, ctypes.POINTER(ctypes.c_int)]),
+    "gz_sketch_merge": (ctypes.c_int, [_vp, _vp]),
+    "gz_sketch_range_merge": (ctypes.c_int, [_vp, _vp, _sz, _sz]),
+    "gz_sketch_merge_raw_bucket_buffer": (ctypes.c_int, [_vp, _vp]),
+    "gz_sketch_zero_contents": (ctypes.c_int, [_vp]),
+    "gz_sketch_reset_sample_state": (ctypes.c_int, [_vp]),
+    "gz_sketch_serialize": (ctypes.c_int, [_vp, _vp, _sz]),
+    "gz_sketch_deserialize": (ctypes.c_int, [_vp, _vp, _sz]),
+    "gz_sketch_equal": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int)]),
+    "gz_sketch_bucket_array_bytes": (_sz, [_vp]),
+    "gz_sketch_get_buckets": (_sz, [_vp]),
+    "gz_sketch_get_columns": (_sz, [_vp]),
+    "gz_sketch_get_num_samples": (_sz, [_vp]),
+    "gz_sketch_get_bkt_per_col": (_sz, [_vp]),
+    "gz_sketch_get_seed": (ctypes.c_uint64, [_vp]),
+    "gz_alg_config_default": (None, [ctypes.POINTER(_AlgConfig)]),
+    "gz_alg_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_AlgConfig), ctypes.POINTER(_vp)]),
+    "gz_alg_destroy": (ctypes.c_int, [_vp]),
+    "gz_alg_allocate_worker_memory": (ctypes.c_int, [_vp, _sz]),
+    "gz_alg_get_desired_updates_per_batch": (_sz, [_vp]),
+    "gz_alg_get_num_vertices": (ctypes.c_uint32, [_vp]),
+    "gz_alg_get_seed": (ctypes.c_uint64, [_vp]),
+    "gz_alg_max_rounds": (_sz, [_vp]),
+    "gz_alg_sketch_bytes": (_sz, [_vp]),
+    "gz_alg_pre_insert": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]),
+    "gz_alg_apply_update_batch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint32, _vp, _sz]),
+    "gz_alg_apply_raw_buckets_update": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp]),
+    "gz_alg_update": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
+    "gz_alg_ingest_pairs_device": (ctypes.c_int, [_vp, _vp, _sz]),
+    "gz_alg_ingest_pairs": (ctypes.c_int, [_vp, _vp, _sz]),
+    "gz_alg_has_cached_query": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "gz_alg_prep_query": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_alg_connected_components": (ctypes.c_int, [_vp, _vp, _u32p]),
+    "gz_alg_point_query": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]),
+    "gz_alg_point_queries": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp]),
+    "gz_alg_calc_spanning_forest": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_sz)]),
+    "gz_alg_write_binary": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "gz_alg_construct_from_serialized_data": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_AlgConfig), ctypes.POINTER(_vp)]),
+    "gz_alg_dump_sketches": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _sz]),
+    "gz_alg_last_query_rounds": (_sz, [_vp]),
+    "gz_alg_get_total_updates": (ctypes.c_uint64, [_vp]),
+    "gz_alg_footprint_bytes": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.POINTER(_AlgConfig)]),
+}
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def lib():
+    """Load libgz_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m graphzeppelin_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _register_extra(L)
+        _lib = L
+    return _lib
+
+
+_EXTRA_SIGS: dict = {}
+
+
+def _register_extra(L):
+    for name, (res, args) in _EXTRA_SIGS.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+
+
+def _check(rc: int):
+    if rc == GZ_OK:
+        return
+    msg = lib().gz_last_error().decode(errors="replace")
+    if rc == GZ_ERR_OUT_OF_SAMPLES:
+        raise OutOfSamplesException(rc, msg)
+    if rc == GZ_ERR_UPDATE_LOCKED:
+        raise UpdateLockedException(rc, msg)
+    if rc == GZ_ERR_ADMISSION:
+        raise AdmissionError(rc, msg)
+    raise GZError(rc, msg)
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _check(lib().gz_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---- geometry (include/sketch.h:62-76, 181) ---------------------------------------------------
+def calc_vector_length(n: int) -> int:
+    return int(lib().gz_calc_vector_length(n))
+
+
+def calc_cc_samples(n: int, f: float = 1.0) -> int:
+    return int(lib().gz_calc_cc_samples(n, f))
+
+
+def calc_bkt_per_col(vector_len: int) -> int:
+    return int(lib().gz_calc_bkt_per_col(vector_len))
+
+
+class Sketch:
+    """include/sketch.h:40-191, buckets resident in HBM."""
+
+    def __init__(self, vector_len: int, seed: int, num_samples: int = 1, cols_per_sample: int = 1,
+                 gamma_bytes: int = 4):
+        h = ctypes.c_void_p()
+        _check(lib().gz_sketch_create(vector_len, seed, num_samples, cols_per_sample, gamma_bytes, ctypes.byref(h)))
+        self._h = h
+        self.gamma_bytes = gamma_bytes
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gz_sketch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, idx: int):
+        _check(lib().gz_sketch_update(self._h, int(idx)))
+
+    def update_batch(self, idx):
+        a = np.ascontiguousarray(np.asarray(idx, dtype=np.uint64))
+        _check(lib().gz_sketch_update_batch(self._h, _ptr(a), a.size))
+
+    def sample(self):
+        i = ctypes.c_uint64()
+        r = ctypes.c_int()
+        _check(lib().gz_sketch_sample(self._h, ctypes.byref(i), ctypes.byref(r)))
+        return int(i.value), int(r.value)
+
+    def merge(self, other: "Sketch"):
+        _check(lib().gz_sketch_merge(self._h, other._h))
+
+    def range_merge(self, other: "Sketch", start_sample: int, n_samples: int):
+        _check(lib().gz_sketch_range_merge(self._h, other._h, start_sample, n_samples))
+
+    def merge_raw_bucket_buffer(self, raw: bytes | np.ndarray):
+        buf = np.frombuffer(bytes(raw), dtype=np.uint8) if not isinstance(raw, np.ndarray) else np.ascontiguousarray(raw)
+        if buf.nbytes < self.bucket_array_bytes():
+            raise ValueError("raw bucket buffer too small")
+        _check(lib().gz_sketch_merge_raw_bucket_buffer(self._h, _ptr(buf)))
+
+    def zero_contents(self):
+        _check(lib().gz_sketch_zero_contents(self._h))
+
+    def reset_sample_state(self):
+        _check(lib().gz_sketch_reset_sample_state(self._h))
+
+    def serialize(self) -> bytes:
+        out = np.empty(self.bucket_array_bytes(), dtype=np.uint8)
+        _check(lib().gz_sketch_serialize(self._h, _ptr(out), out.nbytes))
+        return out.tobytes()
+
+    def deserialize(self, data: bytes):
+        buf = np.frombuffer(data, dtype=np.uint8)
+        _check(lib().gz_sketch_deserialize(self._h, _ptr(buf), buf.nbytes))
+
+    def __eq__(self, other):
+        if not isinstance(other, Sketch):
+            return NotImplemented
+        e = ctypes.c_int()
+        _check(lib().gz_sketch_equal(self._h, other._h, ctypes.byref(e)))
+        return bool(e.value)
+
+    def bucket_array_bytes(self) -> int:
+        return int(lib().gz_sketch_bucket_array_bytes(self._h))
+
+    def get_buckets(self) -> int:
+        return int(lib().gz_sketch_get_buckets(self._h))
+
+    def get_columns(self) -> int:
+        return int(lib().gz_sketch_get_columns(self._h))
+
+    def get_num_samples(self) -> int:
+        return int(lib().gz_sketch_get_num_samples(self._h))
+
+    def get_bkt_per_col(self) -> int:
+        return int(lib().gz_sketch_get_bkt_per_col(self._h))
+
+    def get_seed(self) -> int:
+        return int(lib().gz_sketch_get_seed(self._h))
+
+    def column_seed(self, c: int) -> int:  # include/sketch.h:175
+        return (self.get_seed() + 5 * c) & 0xFFFFFFFFFFFFFFFF
+
+    def checksum_seed(self) -> int:  # include/sketch.h:176
+        return self.get_seed()
+
+
+@dataclass
+class CCAlgConfiguration:
+    """include/cc_alg_configuration.h plus the device-form knobs."""
+    sketches_factor: float = 1.0
+    batch_factor: float = 1.0
+    gamma_bytes: int = 4
+    eager_dsu: bool = True
+    batched: bool = True
+    tile: int = 0
+    flush_threshold: int = 0
+
+    def to_c(self) -> _AlgConfig:
+        c = _AlgConfig()
+        lib().gz_alg_config_default(ctypes.byref(c))
+        c.sketches_factor = self.sketches_factor
+        c.batch_factor = self.batch_factor
+        c.gamma_bytes = self.gamma_bytes
+        c.eager_dsu = 1 if self.eager_dsu else 0
+        c.batched = 1 if self.batched else 0
+        c.tile = self.tile
+        c.flush_threshold = self.flush_threshold
+        return c
+
+
+def footprint_bytes(n: int, config: CCAlgConfiguration | None = None) -> int:
+    c = (config or CCAlgConfiguration()).to_c()
+    return int(lib().gz_alg_footprint_bytes(n, ctypes.byref(c)))
+
+
+class ConnectedComponents:
+    """include/return_types.h:13-27: parent array of DSU roots and the component count."""
+
+    def __init__(self, parents: np.ndarray, num_cc: int):
+        self.parent_arr = parents
+        self.num_cc = num_cc
+
+    def size(self) -> int:
+        return self.num_cc
+
+    def is_connected(self, a: int, b: int) -> bool:
+        return bool(self.parent_arr[a] == self.parent_arr[b])
+
+    def get_component_sets(self) -> list[set[int]]:
+        order = np.argsort(self.parent_arr, kind="stable")
+        roots = self.parent_arr[order]
+        cuts = np.flatnonzero(np.diff(roots)) + 1
+        return [set(map(int, grp)) for grp in np.split(order, cuts)]
+
+    def canonical_labels(self) -> np.ndarray:
+        """Smallest vertex id of each vertex's component (root-independent form)."""
+        lab = np.full(self.parent_arr.size, np.iinfo(np.uint32).max, dtype=np.uint32)
+        np.minimum.at(lab, self.parent_arr, np.arange(self.parent_arr.size, dtype=np.uint32))
+        return lab[self.parent_arr]
+
+
+class CCSketchAlg:
+    """include/cc_sketch_alg.h:67-257 with every vertex sketch resident in HBM."""
+
+    def __init__(self, num_vertices: int, seed: int, config: CCAlgConfiguration | None = None, _handle=None):
+        self.config = config or CCAlgConfiguration()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            h = ctypes.c_void_p()
+            c = self.config.to_c()
+            _check(lib().gz_alg_create(num_vertices, seed, ctypes.byref(c), ctypes.byref(h)))
+            self._h = h
+        self.num_vertices = int(lib().gz_alg_get_num_vertices(self._h))
+
+    @classmethod
+    def construct_from_serialized_data(cls, path: str, config: CCAlgConfiguration | None = None):
+        config = config or CCAlgConfiguration()
+        h = ctypes.c_void_p()
+        c = config.to_c()
+        _check(lib().gz_alg_construct_from_serialized_data(path.encode(), ctypes.byref(c), ctypes.byref(h)))
+        return cls(0, 0, config, _handle=h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gz_alg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # plugin interface (include/graph_sketch_driver.h:23-69)
+    def allocate_worker_memory(self, num_workers: int):
+        _check(lib().gz_alg_allocate_worker_memory(self._h, num_workers))
+
+    def get_desired_updates_per_batch(self) -> int:
+        return int(lib().gz_alg_get_desired_updates_per_batch(self._h))
+
+    def get_num_vertices(self) -> int:
+        return self.num_vertices
+
+    def get_seed(self) -> int:
+        return int(lib().gz_alg_get_seed(self._h))
+
+    def max_rounds(self) -> int:
+        return int(lib().gz_alg_max_rounds(self._h))
+
+    def sketch_bytes(self) -> int:
+        return int(lib().gz_alg_sketch_bytes(self._h))
+
+    def pre_insert(self, src: int, dst: int, type_: int = INSERT, thr_id: int = 0):
+        _check(lib().gz_alg_pre_insert(self._h, src, dst, type_, thr_id))
+
+    def apply_update_batch(self, thr_id: int, src: int, dsts):
+        a = np.ascontiguousarray(np.asarray(dsts, dtype=np.uint32))
+        _check(lib().gz_alg_apply_update_batch(self._h, thr_id, src, _ptr(a), a.size))
+
+    def apply_raw_buckets_update(self, src: int, raw):
+        buf = np.ascontiguousarray(np.frombuffer(bytes(raw), dtype=np.uint8))
+        if buf.nbytes < self.sketch_bytes():
+            raise ValueError("raw bucket buffer too small")
+        _check(lib().gz_alg_apply_raw_buckets_update(self._h, src, _ptr(buf)))
+
+    def update(self, src: int, dst: int, type_: int = INSERT):
+        _check(lib().gz_alg_update(self._h, src, dst, type_))
+
+    def ingest_pairs(self, pairs: np.ndarray):
+        """(src,dst) uint32 pairs in host memory, shape (count, 2)."""
+        a = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint32).reshape(-1, 2))
+        _check(lib().gz_alg_ingest_pairs(self._h, _ptr(a), a.shape[0]))
+
+    def ingest_pairs_device(self, ptr: int, count: int):
+        """(src,dst) uint32 pairs already in device memory (e.g. a torch tensor's data_ptr())."""
+        _check(lib().gz_alg_ingest_pairs_device(self._h, ctypes.c_void_p(ptr), count))
+
+    def has_cached_query(self, code: int = CONNECTIVITY) -> bool:
+        r = ctypes.c_int()
+        _check(lib().gz_alg_has_cached_query(self._h, code, ctypes.byref(r)))
+        return bool(r.value)
+
+    def prep_query(self, code: int = CONNECTIVITY):
+        _check(lib().gz_alg_prep_query(self._h, code))
+
+    def connected_components(self) -> ConnectedComponents:
+        par = np.empty(self.num_vertices, dtype=np.uint32)
+        ncc = ctypes.c_uint32()
+        _check(lib().gz_alg_connected_components(self._h, _ptr(par), ctypes.byref(ncc)))
+        return ConnectedComponents(par, int(ncc.value))
+
+    def point_query(self, a: int, b: int) -> bool:
+        r = ctypes.c_int()
+        _check(lib().gz_alg_point_query(self._h, a, b, ctypes.byref(r)))
+        return bool(r.value)
+
+    def point_queries(self, us, vs) -> np.ndarray:
+        u = np.ascontiguousarray(np.asarray(us, dtype=np.uint32))
+        v = np.ascontiguousarray(np.asarray(vs, dtype=np.uint32))
+        out = np.empty(u.size, dtype=np.uint8)
+        _check(lib().gz_alg_point_queries(self._h, _ptr(u), _ptr(v), u.size, _ptr(out)))
+        return out.astype(bool)
+
+    def calc_spanning_forest(self) -> np.ndarray:
+        cap = max(1, self.num_vertices)
+        out = np.empty((cap, 2), dtype=np.uint32)
+        k = ctypes.c_size_t()
+        _check(lib().gz_alg_calc_spanning_forest(self._h, _ptr(out), cap, ctypes.byref(k)))
+        return out[: min(k.value, cap)].copy()
+
+    def write_binary(self, path: str):
+        _check(lib().gz_alg_write_binary(self._h, path.encode()))
+
+    def dump_sketches(self, v0: int = 0, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = self.num_vertices - v0
+        sb = self.sketch_bytes()
+        out = np.empty(count * sb, dtype=np.uint8)
+        _check(lib().gz_alg_dump_sketches(self._h, v0, count, _ptr(out), out.nbytes))
+        return out
+
+    @property
+    def last_query_rounds(self) -> int:
+        return int(lib().gz_alg_last_query_rounds(self._h))
+
+    def get_total_updates(self) -> int:
+        return int(lib().gz_alg_get_total_updates(self._h))

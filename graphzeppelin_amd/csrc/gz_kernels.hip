@@ -1,0 +1,918 @@
+// gz_kernels.hip -- hand-written gfx950 kernels for GraphZeppelin's streaming-CC hot path.
+//
+// Data layout in HBM (one "slab" per algorithm instance):
+//   alpha plane  u64 [n][stride]   bucket b of vertex v at alpha[v*stride + b]
+//   gamma plane  G   [n][stride]   G = u32 (12-byte reference bucket) or u64 (16-byte bucket)
+//   bucket b = c*D + d for column c, depth d; the deterministic bucket is b = C*D (last), exactly
+//   the reference's bucket order (src/sketch.cpp:69-83), so packing the two planes back into
+//   {alpha, gamma} records reproduces the reference's byte image.
+// Everything is integer work; nothing here is a dense contraction, so no MFMA is used.
+#include <hip/hip_runtime.h>
+
+#include "gz_kernels.hpp"
+
+namespace gz {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+__device__ inline uint32_t rd_lane_u32(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ inline uint64_t rd_lane_u64(uint64_t v, int lane) {
+  uint32_t lo = rd_lane_u32((uint32_t)v, lane), hi = rd_lane_u32((uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline uint64_t shfl_u64(uint64_t v, int src) {
+  uint32_t lo = __shfl((unsigned)(uint32_t)v, src), hi = __shfl((unsigned)(uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline uint64_t wave_xor_u64(uint64_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = __shfl_xor((unsigned)(uint32_t)v, o), hi = __shfl_xor((unsigned)(uint32_t)(v >> 32), o);
+    v ^= ((uint64_t)hi << 32) | lo;
+  }
+  return v;
+}
+__device__ inline uint64_t wave_incl_scan_u64(uint64_t x) {
+  const uint32_t lane = lane_id();
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t lo = __shfl_up((unsigned)(uint32_t)x, o), hi = __shfl_up((unsigned)(uint32_t)(x >> 32), o);
+    uint64_t y = ((uint64_t)hi << 32) | lo;
+    if (lane >= (uint32_t)o) x += y;
+  }
+  return x;
+}
+
+template <class T>
+__device__ inline void lds_xor(T* p, T v) {
+  __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ inline void gbl_xor(T* p, T v) {
+  __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline T ld_acq(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void st_rlx(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- DSU (include/dsu.h:101-174 restated for the device) --------------------------------------
+// Priorities are a bijective 32-bit mix of the vertex id, so (priority, id) is a strict total
+// order and linking the lower-priority root under the higher one can never form a cycle.
+__device__ inline uint32_t dsu_prio(uint32_t v, uint32_t seed) {
+  uint32_t h = v ^ seed;
+  h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+  return h;
+}
+__device__ inline uint32_t dsu_find(uint32_t* parent, uint32_t u) {
+  // path halving, dsu.h:145-152
+  uint32_t p = ld_acq(parent + u);
+  while (true) {
+    uint32_t gp = ld_acq(parent + p);
+    if (gp == p) return p;
+    st_rlx(parent + u, gp);
+    u = gp;
+    p = ld_acq(parent + u);
+  }
+}
+__device__ inline bool dsu_merge(uint32_t* parent, uint32_t a, uint32_t b, uint32_t pseed) {
+  // CAS union ordered by priority, dsu.h:155-167
+  while (true) {
+    a = dsu_find(parent, a);
+    b = dsu_find(parent, b);
+    if (a == b) return false;
+    uint32_t pa = dsu_prio(a, pseed), pb = dsu_prio(b, pseed);
+    if (pa == pb && a > b) { uint32_t t = a; a = b; b = t; }
+    if (pa < pb) { uint32_t t = a; a = b; b = t; }
+    uint32_t expected = b;
+    if (__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return true;
+  }
+}
+
+// ---- spanning-forest edge set (open addressing on 64-bit edge ids) ----------------------------
+constexpr uint64_t kEmpty = ~0ull;
+__device__ inline uint64_t mix64(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+  return k;
+}
+__device__ inline void sf_insert(const DsuDev& d, uint64_t key) {
+  uint64_t h = mix64(key) & d.sf_mask;
+  for (uint64_t probe = 0; probe <= d.sf_mask; ++probe) {
+    uint64_t expected = kEmpty;
+    if (__hip_atomic_compare_exchange_strong(d.sf_hash + h, &expected, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) ||
+        expected == key)
+      return;
+    h = (h + 1) & d.sf_mask;
+  }
+}
+__device__ inline bool sf_contains(const DsuDev& d, uint64_t key) {
+  uint64_t h = mix64(key) & d.sf_mask;
+  for (uint64_t probe = 0; probe <= d.sf_mask; ++probe) {
+    uint64_t cur = ld_acq(d.sf_hash + h);
+    if (cur == key) return true;
+    if (cur == kEmpty) return false;
+    h = (h + 1) & d.sf_mask;
+  }
+  return false;
+}
+// A successful merge records the edge in the spanning forest (cc_sketch_alg.cpp:152-162)
+__device__ inline void sf_record(const DsuDev& d, uint32_t u, uint32_t v) {
+  uint32_t a = u < v ? u : v, b = u < v ? v : u;
+  sf_insert(d, ((uint64_t)a << 32) | b);
+  uint32_t k = __hip_atomic_fetch_add(d.flags + kFlagSfCount, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (k < d.n) { d.sf_list[2 * k] = a; d.sf_list[2 * k + 1] = b; }
+}
+
+// ---- sample (src/sketch.cpp:94-116) over one wave --------------------------------------------
+// Lane l < D holds bucket (column, depth l); every lane holds the deterministic bucket.
+// Returns the code; *idx gets the sampled index on GOOD.
+template <class G>
+__device__ inline int wave_sample(uint64_t det_a, G det_g, uint64_t a, G gm, bool lane_valid, uint64_t chk_bf,
+                                  uint64_t* idx) {
+  if (det_a == 0 && det_g == 0) { *idx = 0; return kZero; }
+  if ((G)xxh3_mix(xxh3_prep(det_a), chk_bf) == det_g) { *idx = det_a; return kGood; }
+  bool good = lane_valid && (G)xxh3_mix(xxh3_prep(a), chk_bf) == gm;
+  uint64_t mask = __ballot(good);
+  if (mask) {
+    int first = __builtin_ctzll(mask);
+    *idx = rd_lane_u64(a, first);
+    return kGood;
+  }
+  *idx = 0;
+  return kFail;
+}
+
+// Act on a supernode's sample (sample_supernode, src/cc_sketch_alg.cpp:137-166).  Lane 0 only.
+__device__ inline void act_on_sample(const DsuDev& d, int code, uint64_t idx) {
+  if (code == kFail) { st_rlx(d.flags + kFlagModified, 1u); return; }
+  if (code != kGood) return;
+  uint32_t u = (uint32_t)(idx >> 32), v = (uint32_t)idx;
+  if (u >= d.n || v >= d.n) {
+    // index decoded from a checksum collision; counted as a failed sample
+    st_rlx(d.flags + kFlagModified, 1u);
+    st_rlx(d.flags + kFlagBad, 1u);
+    return;
+  }
+  if (dsu_merge(d.parent, u, v, d.prio_seed)) {
+    sf_record(d, u, v);
+    st_rlx(d.flags + kFlagModified, 1u);
+  }
+}
+
+// ================================================================================================
+// fills
+// ================================================================================================
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ================================================================================================
+// Sketch update: Sketch::update over a batch (src/sketch.cpp:69-83) fused with the delta merge
+// (src/cc_sketch_alg.cpp:106-118, src/sketch.cpp:149-154).
+//
+// One workgroup per tile (<= tile-size destinations of one vertex).  The tile's delta sketch
+// lives in LDS; lane l starts at column l % C and walks the columns cyclically, so the lanes of a
+// wave XOR into different columns at the same time (column pitch Dp = 1 mod 16 spreads them over
+// the LDS banks) instead of all hitting depth 0 of one column.  The deterministic bucket is kept
+// in registers per lane and reduced once per tile.  The write-back is a coalesced
+// read-xor-write of the vertex's bucket planes, or device-scope atomic XOR when the vertex is
+// split across tiles of the same launch.  The grid is persistent: workgroups pull tiles from an
+// atomic counter until the counter passes the tile count.
+// ================================================================================================
+template <class G, bool RAW, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntiles_dev, uint32_t ntiles_host,
+               uint32_t* __restrict__ work_counter, const uint32_t* __restrict__ dsts,
+               const uint64_t* __restrict__ raw_idx, uint64_t* __restrict__ alpha, G* __restrict__ gamma, Geo g,
+               const uint64_t* __restrict__ col_bf) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t C = g.C, D = g.D, Dp = g.Dp;
+  const uint32_t lds_n = C * Dp + 1;  // + deterministic bucket at C*Dp
+  uint64_t* s_bf = reinterpret_cast<uint64_t*>(smem);
+  uint64_t* s_a = s_bf + C;
+  G* s_g = reinterpret_cast<G*>(s_a + lds_n);
+  __shared__ uint32_t s_tile;
+
+  for (uint32_t i = threadIdx.x; i < C; i += BLOCK) s_bf[i] = col_bf[i];
+  const uint32_t ntiles = ntiles_dev ? *ntiles_dev : ntiles_host;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t start_col = lane % C;
+  const uint64_t depth_cap = 1ull << D;
+
+  // work_counter == nullptr: static mapping, workgroup b processes tile b only
+  bool first = true;
+  while (true) {
+    if (threadIdx.x == 0) s_tile = work_counter ? atomicAdd(work_counter, 1u) : (first ? blockIdx.x : ntiles);
+    first = false;
+    for (uint32_t i = threadIdx.x; i < lds_n; i += BLOCK) { s_a[i] = 0; s_g[i] = 0; }
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= ntiles) break;
+    const Tile tl = tiles[t];
+
+    uint64_t det_a = 0;
+    G det_g = 0;
+    for (uint32_t i = threadIdx.x; i < tl.count; i += BLOCK) {
+      const uint64_t idx = RAW ? raw_idx[tl.begin + i] : concat_pairing(tl.vertex, dsts[tl.begin + i]);
+      const uint64_t pre = xxh3_prep(idx);
+      const G chk = (G)xxh3_mix(pre, g.chk_bf);
+      det_a ^= idx;
+      det_g ^= chk;
+      uint32_t c = start_col;
+      for (uint32_t s = 0; s < C; ++s) {
+        const uint64_t h = xxh3_mix(pre, s_bf[c]);
+        const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);
+        if (depth < D) {
+          const uint32_t li = c * Dp + depth;
+          lds_xor(s_a + li, idx);
+          lds_xor(s_g + li, chk);
+        }
+        c = (c + 1 == C) ? 0 : c + 1;
+      }
+    }
+    det_a = wave_xor_u64(det_a);
+    det_g = (G)wave_xor_u64((uint64_t)det_g);
+    if (lane == 0 && (det_a | (uint64_t)det_g)) {
+      lds_xor(s_a + C * Dp, det_a);
+      lds_xor(s_g + C * Dp, det_g);
+    }
+    __syncthreads();
+
+    // write-back of the delta into the vertex's planes
+    const size_t base = (size_t)tl.vertex * g.stride;
+    const uint32_t nb = C * D + 1;
+    for (uint32_t b = threadIdx.x; b < nb; b += BLOCK) {
+      uint32_t li;
+      if (b == C * D) {
+        li = C * Dp;
+      } else {
+        const uint32_t c = b / D;
+        li = c * Dp + (b - c * D);
+      }
+      const uint64_t da = s_a[li];
+      const G dg = s_g[li];
+      if ((da | (uint64_t)dg) == 0) continue;
+      if (tl.split) {
+        if (da) gbl_xor(alpha + base + b, da);
+        if (dg) gbl_xor(gamma + base + b, dg);
+      } else {
+        alpha[base + b] ^= da;
+        gamma[base + b] ^= dg;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ================================================================================================
+// Grouping a stream of (src,dst) pairs by vertex: the device form of the reference's gutters
+// (both directions of every update, include/graph_sketch_driver.h:171-172).
+// ================================================================================================
+// Pairs naming a vertex >= n are skipped by every kernel and reported through bad_flag.
+__global__ void k_count_pairs(const uint2* __restrict__ pairs, size_t count, uint32_t n, uint32_t* __restrict__ cnt,
+                              uint32_t* __restrict__ bad_flag) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const uint2 p = pairs[i];
+    if (p.x >= n || p.y >= n) {
+      st_rlx(bad_flag, 1u);
+      continue;
+    }
+    atomicAdd(cnt + p.x, 1u);
+    atomicAdd(cnt + p.y, 1u);
+  }
+}
+__global__ void k_scatter_pairs(const uint2* __restrict__ pairs, size_t count, uint32_t n,
+                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ dst_arr) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const uint2 p = pairs[i];
+    if (p.x >= n || p.y >= n) continue;
+    const uint32_t pu = atomicAdd(cursor + p.x, 1u);
+    dst_arr[pu] = p.y;
+    const uint32_t pv = atomicAdd(cursor + p.y, 1u);
+    dst_arr[pv] = p.x;
+  }
+}
+
+// Exclusive scan of per-vertex counts (and of per-vertex tile counts ceil(cnt/tile) when tile>0),
+// packed as (tiles << 32 | cnt) in one u64 so a single pass scans both.
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr uint32_t kScanSpan = kScanBlock * kScanItems;
+
+__device__ inline uint64_t pack_ct(uint32_t c, uint32_t tile) {
+  const uint64_t nt = tile ? (c + tile - 1) / tile : 0;
+  return (nt << 32) | c;
+}
+__device__ inline uint64_t block_excl_scan_u64(uint64_t x, uint64_t* total) {
+  __shared__ uint64_t wsum[kScanBlock / kWave];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  const uint64_t incl = wave_incl_scan_u64(x);
+  if (lane == kWave - 1) wsum[wid] = incl;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+  for (uint32_t w = 0; w < kScanBlock / kWave; ++w) {
+    if (w < wid) pre += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + incl - x;
+}
+__global__ void __launch_bounds__(kScanBlock)
+k_scan_partials(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t tile, uint64_t* __restrict__ partials) {
+  const uint32_t base = blockIdx.x * kScanSpan + threadIdx.x * kScanItems;
+  uint64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t i = base + k;
+    if (i < n) s += pack_ct(cnt[i], tile);
+  }
+  uint64_t tot;
+  block_excl_scan_u64(s, &tot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(kScanBlock)
+k_scan_block_sums(uint64_t* __restrict__ partials, uint32_t nparts, uint32_t n, uint32_t* __restrict__ off,
+                  uint32_t* __restrict__ toff, uint32_t* __restrict__ totals) {
+  const uint32_t per = (nparts + kScanBlock - 1) / kScanBlock;
+  const uint32_t b0 = threadIdx.x * per;
+  uint64_t s = 0;
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nparts) s += partials[b0 + k];
+  uint64_t tot;
+  uint64_t run = block_excl_scan_u64(s, &tot);
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nparts) {
+      const uint64_t v = partials[b0 + k];
+      partials[b0 + k] = run;
+      run += v;
+    }
+  if (threadIdx.x == 0) {
+    totals[0] = (uint32_t)tot;
+    totals[1] = (uint32_t)(tot >> 32);
+    off[n] = (uint32_t)tot;
+    if (toff) toff[n] = (uint32_t)(tot >> 32);
+  }
+}
+__global__ void __launch_bounds__(kScanBlock)
+k_scan_apply(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t tile, const uint64_t* __restrict__ partials,
+             uint32_t* __restrict__ off, uint32_t* __restrict__ toff, uint32_t* __restrict__ cursor) {
+  const uint32_t base = blockIdx.x * kScanSpan + threadIdx.x * kScanItems;
+  uint32_t c[kScanItems];
+  uint64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t i = base + k;
+    c[k] = i < n ? cnt[i] : 0;
+    s += pack_ct(c[k], tile);
+  }
+  uint64_t tot;
+  uint64_t run = block_excl_scan_u64(s, &tot) + partials[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t i = base + k;
+    if (i < n) {
+      off[i] = (uint32_t)run;
+      if (toff) toff[i] = (uint32_t)(run >> 32);
+      if (cursor) cursor[i] = (uint32_t)run;
+    }
+    run += pack_ct(c[k], tile);
+  }
+}
+__global__ void k_make_tiles(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
+                             const uint32_t* __restrict__ toff, uint32_t n, uint32_t tile, Tile* __restrict__ tiles) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t c = cnt[v];
+    if (!c) continue;
+    const uint32_t nt = (c + tile - 1) / tile, o = off[v], t0 = toff[v];
+    for (uint32_t j = 0; j < nt; ++j) {
+      const uint32_t b = j * tile;
+      tiles[t0 + j] = Tile{v, o + b, (c - b) < tile ? (c - b) : tile, nt > 1 ? 1u : 0u};
+    }
+  }
+}
+
+void scan_counts(const uint32_t* cnt, uint32_t n, uint32_t tile, uint64_t* partials, uint32_t* off, uint32_t* toff,
+                 uint32_t* cursor, uint32_t* totals, hipStream_t s) {
+  const uint32_t nparts = (n + kScanSpan - 1) / kScanSpan;
+  hipLaunchKernelGGL(k_scan_partials, dim3(nparts), dim3(kScanBlock), 0, s, cnt, n, tile, partials);
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanBlock), 0, s, partials, nparts, n, off, toff, totals);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nparts), dim3(kScanBlock), 0, s, cnt, n, tile, partials, off, toff, cursor);
+}
+
+inline uint32_t grid_for(size_t work, uint32_t block, uint32_t cap = 8192) {
+  size_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  return (uint32_t)(g < cap ? g : cap);
+}
+
+// ================================================================================================
+// Eager DSU over a batch of updates (pre_insert, src/cc_sketch_alg.cpp:79-104), in three phases
+// separated by kernel boundaries:
+//   1. every update attempts the CAS union of its endpoints; winners are marked merged;
+//   2. merged edges are inserted into the spanning-forest set;
+//   3. an update that did not merge and whose edge is in the forest invalidates the DSU.
+// The outcome equals the reference's for the linearisation the CAS order defines (the
+// reference's stream threads are concurrent as well): a deleted forest edge always invalidates,
+// and any edge that merged and occurs again in the batch invalidates.
+// ================================================================================================
+__global__ void k_eager_union(const uint2* __restrict__ pairs, size_t count, uint8_t* __restrict__ merged, DsuDev d) {
+  if (ld_acq(d.flags + kFlagEagerInvalid)) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const uint2 p = pairs[i];
+    if (p.x >= d.n || p.y >= d.n) {
+      merged[i] = 0;
+      st_rlx(d.flags + kFlagBadInput, 1u);
+      continue;
+    }
+    merged[i] = dsu_merge(d.parent, p.x, p.y, d.prio_seed) ? 1 : 0;
+  }
+}
+__global__ void k_eager_sf_insert(const uint2* __restrict__ pairs, size_t count, const uint8_t* __restrict__ merged,
+                                  DsuDev d) {
+  if (ld_acq(d.flags + kFlagEagerInvalid)) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    if (merged[i]) sf_record(d, pairs[i].x, pairs[i].y);
+}
+__global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, const uint8_t* __restrict__ merged,
+                                 DsuDev d) {
+  if (ld_acq(d.flags + kFlagEagerInvalid)) return;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    if (merged[i]) continue;
+    const uint2 p = pairs[i];
+    if (p.x == p.y || p.x >= d.n || p.y >= d.n) continue;
+    if (sf_contains(d, concat_pairing(p.x, p.y))) st_rlx(d.flags + kFlagEagerInvalid, 1u);
+  }
+}
+
+// ================================================================================================
+// Borůvka (boruvka_emulation, src/cc_sketch_alg.cpp:464-513)
+// ================================================================================================
+__global__ void k_dsu_reset(uint32_t* parent, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) parent[i] = i;
+}
+
+// Round 0 (run_round_zero, 223-244): every vertex samples column 0 of its own sketch.  One wave
+// per vertex; lane l checks depth l, the first good depth is the lowest set bit of the ballot.
+template <class G>
+__global__ void __launch_bounds__(256)
+k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, DsuDev d) {
+  const uint32_t lane = lane_id();
+  const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < g.n; v += waves) {
+    const size_t base = (size_t)v * g.stride;
+    const uint64_t det_a = alpha[base + g.nb - 1];
+    const G det_g = gamma[base + g.nb - 1];
+    const bool valid = lane < g.D;
+    const uint64_t a = valid ? alpha[base + lane] : 0;
+    const G gm = valid ? gamma[base + lane] : 0;
+    uint64_t idx;
+    const int code = wave_sample<G>(det_a, det_g, a, gm, valid, g.chk_bf, &idx);
+    if (lane == 0) act_on_sample(d, code, idx);
+  }
+}
+
+// Root of every vertex (create_merge_instructions' find_root pass, 382-403)
+__global__ void k_compress(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    root[v] = dsu_find(parent, v);
+}
+
+// Wave-aggregated counting sort by root: lanes that share a root issue one atomic together.
+__global__ void k_count_roots(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cnt) {
+  const uint32_t lane = lane_id();
+  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + lane;
+    const bool active = v < n;
+    const uint32_t r = active ? root[v] : 0;
+    uint64_t todo = __ballot(active);
+    while (todo) {
+      const int leader = __builtin_ctzll(todo);
+      const uint32_t lr = rd_lane_u32(r, leader);
+      const uint64_t same = __ballot(active && r == lr) & todo;
+      if ((int)lane == leader) atomicAdd(cnt + lr, (uint32_t)__popcll(same));
+      todo &= ~same;
+    }
+  }
+}
+__global__ void k_scatter_roots(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
+                                uint32_t* __restrict__ members) {
+  const uint32_t lane = lane_id();
+  const uint64_t lt = (1ull << lane) - 1;
+  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + lane;
+    const bool active = v < n;
+    const uint32_t r = active ? root[v] : 0;
+    uint64_t todo = __ballot(active);
+    while (todo) {
+      const int leader = __builtin_ctzll(todo);
+      const uint32_t lr = rd_lane_u32(r, leader);
+      const uint64_t same = __ballot(active && r == lr) & todo;
+      uint32_t b = 0;
+      if ((int)lane == leader) b = atomicAdd(cursor + lr, (uint32_t)__popcll(same));
+      b = __shfl(b, leader);
+      if ((same >> lane) & 1) members[b + __popcll(same & lt)] = v;
+      todo &= ~same;
+    }
+  }
+}
+
+// Rounds >= 1 (perform_boruvka_round, 246-380, with range_merge, src/sketch.cpp:156-179):
+// the members array is grouped by root; each wave owns kWave consecutive positions.  Lane l < D
+// accumulates depth l of column `round`, lane D the deterministic bucket.  A supernode that lies
+// inside one wave's range is sampled straight from registers; one that spans several ranges is
+// XOR-ed into an accumulator slot with device atomics and sampled by the last wave to arrive.
+template <class G>
+__device__ inline void finalize_segment(uint32_t R, uint64_t acc_a, G acc_g, uint32_t p0, uint32_t p1,
+                                        const uint32_t* off, const uint32_t* cnt, const Geo& g, const DsuDev& d,
+                                        uint64_t* slot_a, uint64_t* slot_g, uint32_t* done) {
+  const uint32_t lane = lane_id();
+  const uint32_t s = off[R], e = s + cnt[R];
+  const bool valid = lane < g.D;
+  if (s >= p0 && e <= p1) {
+    const uint64_t det_a = rd_lane_u64(acc_a, g.D);
+    const G det_g = (G)rd_lane_u64((uint64_t)acc_g, g.D);
+    uint64_t idx;
+    const int code = wave_sample<G>(det_a, det_g, acc_a, acc_g, valid, g.chk_bf, &idx);
+    if (lane == 0) act_on_sample(d, code, idx);
+    return;
+  }
+  const uint32_t first = s / kWave, last = (e - 1) / kWave, parts = last - first + 1;
+  uint64_t* sa = slot_a + (size_t)first * kWave;
+  uint64_t* sg = slot_g + (size_t)first * kWave;
+  if (lane <= g.D) {
+    if (acc_a) gbl_xor(sa + lane, acc_a);
+    if (acc_g) gbl_xor(sg + lane, (uint64_t)acc_g);
+  }
+  __threadfence();
+  uint32_t old = 0;
+  if (lane == 0) old = atomicAdd(done + first, 1u);
+  old = rd_lane_u32(old, 0);
+  if (old != parts - 1) return;
+  __threadfence();
+  const uint64_t ta = lane <= g.D ? ld_acq(sa + lane) : 0;
+  const G tg = lane <= g.D ? (G)ld_acq(sg + lane) : 0;
+  const uint64_t det_a = rd_lane_u64(ta, g.D);
+  const G det_g = (G)rd_lane_u64((uint64_t)tg, g.D);
+  uint64_t idx;
+  const int code = wave_sample<G>(det_a, det_g, ta, tg, valid, g.chk_bf, &idx);
+  if (lane == 0) act_on_sample(d, code, idx);
+}
+
+template <class G>
+__global__ void __launch_bounds__(256)
+k_reduce_sample(const uint32_t* __restrict__ members, const uint32_t* __restrict__ root,
+                const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t round,
+                const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, DsuDev d,
+                uint64_t* __restrict__ slot_a, uint64_t* __restrict__ slot_g, uint32_t* __restrict__ done) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t p0 = wave * kWave;
+  if (p0 >= g.n) return;
+  const uint32_t p1 = (p0 + kWave < g.n) ? p0 + kWave : g.n;
+  const uint32_t len = p1 - p0;
+  const uint32_t m = lane < len ? members[p0 + lane] : 0;
+  const uint32_t rt = lane < len ? root[m] : 0;
+  // lane -> bucket offset within the vertex: depth `lane` of column `round`, or the det bucket
+  const bool in_col = lane < g.D;
+  const bool is_det = lane == g.D;
+  const uint32_t boff = in_col ? round * g.D + lane : (g.nb - 1);
+  const bool loads = in_col || is_det;
+
+  uint64_t acc_a = 0;
+  G acc_g = 0;
+  uint32_t cur = rd_lane_u32(rt, 0);
+  constexpr uint32_t U = 4;
+  for (uint32_t j = 0; j < len; j += U) {
+    uint64_t va[U];
+    G vg[U];
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k) {
+      va[k] = 0;
+      vg[k] = 0;
+      if (j + k < len) {
+        const uint32_t v = rd_lane_u32(m, j + k);
+        if (loads) {
+          va[k] = alpha[(size_t)v * g.stride + boff];
+          vg[k] = gamma[(size_t)v * g.stride + boff];
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k) {
+      if (j + k < len) {
+        const uint32_t R = rd_lane_u32(rt, j + k);
+        if (R != cur) {
+          finalize_segment<G>(cur, acc_a, acc_g, p0, p1, off, cnt, g, d, slot_a, slot_g, done);
+          acc_a = 0;
+          acc_g = 0;
+          cur = R;
+        }
+        acc_a ^= va[k];
+        acc_g ^= vg[k];
+      }
+    }
+  }
+  finalize_segment<G>(cur, acc_a, acc_g, p0, p1, off, cnt, g, d, slot_a, slot_g, done);
+}
+
+// ConnectedComponents (src/return_types.cpp:5-19): root of every vertex and the root count.
+__global__ void k_labels(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ labels,
+                         uint32_t* __restrict__ ncc) {
+  const uint32_t lane = lane_id();
+  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + lane;
+    bool is_root = false;
+    if (v < n) {
+      const uint32_t r = dsu_find(parent, v);
+      labels[v] = r;
+      is_root = r == v;
+    }
+    const uint64_t m = __ballot(is_root);
+    if (lane == 0 && m) atomicAdd(ncc, (uint32_t)__popcll(m));
+  }
+}
+__global__ void k_point_queries(uint32_t* __restrict__ parent, const uint32_t* __restrict__ qa,
+                                const uint32_t* __restrict__ qb, uint32_t nq, uint8_t* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
+    out[i] = dsu_find(parent, qa[i]) == dsu_find(parent, qb[i]) ? 1 : 0;
+}
+
+// ================================================================================================
+// Standalone sketch ops
+// ================================================================================================
+// Sketch::sample for any cols_per_sample: deterministic bucket first, then the sample's columns
+// in bucket order; one wave walks them 64 buckets at a time.
+template <class G>
+__global__ void k_sketch_sample(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g,
+                                uint32_t sample_idx, uint32_t* __restrict__ out) {
+  const uint32_t lane = lane_id();
+  const uint64_t det_a = alpha[g.nb - 1];
+  const G det_g = gamma[g.nb - 1];
+  int code;
+  uint64_t idx = 0;
+  if (det_a == 0 && det_g == 0) {
+    code = kZero;
+  } else if ((G)xxh3_mix(xxh3_prep(det_a), g.chk_bf) == det_g) {
+    code = kGood;
+    idx = det_a;
+  } else {
+    code = kFail;
+    const uint32_t b0 = sample_idx * g.cps * g.D, b1 = b0 + g.cps * g.D;
+    for (uint32_t b = b0; b < b1; b += kWave) {
+      const bool valid = b + lane < b1;
+      const uint64_t a = valid ? alpha[b + lane] : 0;
+      const G gm = valid ? gamma[b + lane] : 0;
+      const bool good = valid && (G)xxh3_mix(xxh3_prep(a), g.chk_bf) == gm;
+      const uint64_t mask = __ballot(good);
+      if (mask) {
+        idx = rd_lane_u64(a, __builtin_ctzll(mask));
+        code = kGood;
+        break;
+      }
+    }
+  }
+  if (lane == 0) {
+    out[0] = (uint32_t)code;
+    out[1] = (uint32_t)idx;
+    out[2] = (uint32_t)(idx >> 32);
+  }
+}
+
+template <class G>
+__global__ void k_xor_range(uint64_t* __restrict__ da, G* __restrict__ dg, const uint64_t* __restrict__ sa,
+                            const G* __restrict__ sg, Geo g, uint32_t nsk, uint32_t b0, uint32_t cnt, int with_det) {
+  const uint32_t per = cnt + (with_det ? 1 : 0);
+  const size_t total = (size_t)nsk * per;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)(i / per), k = (uint32_t)(i % per);
+    const uint32_t b = k < cnt ? b0 + k : g.nb - 1;
+    const size_t o = (size_t)s * g.stride + b;
+    da[o] ^= sa[o];
+    dg[o] ^= sg[o];
+  }
+}
+
+// Packed reference image: record i of sketch v holds {u64 alpha; G gamma} at byte
+// (v * nb + i) * (8 + sizeof(G)).  A 12-byte record is only 4-byte aligned, so it is moved as
+// three 32-bit words.
+template <class G>
+__device__ inline void load_packed(const unsigned char* p, uint64_t* a, G* gm) {
+  if (sizeof(G) == 4) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+    *a = ((uint64_t)w[1] << 32) | w[0];
+    *gm = (G)w[2];
+  } else {
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(p);
+    *a = w[0];
+    *gm = (G)w[1];
+  }
+}
+template <class G>
+__device__ inline void store_packed(unsigned char* p, uint64_t a, G gm) {
+  if (sizeof(G) == 4) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(p);
+    w[0] = (uint32_t)a;
+    w[1] = (uint32_t)(a >> 32);
+    w[2] = (uint32_t)gm;
+  } else {
+    uint64_t* w = reinterpret_cast<uint64_t*>(p);
+    w[0] = a;
+    w[1] = (uint64_t)gm;
+  }
+}
+template <class G, int MODE>  // 0 = xor into planes, 1 = overwrite planes, 2 = pack planes
+__global__ void k_packed(uint64_t* __restrict__ alpha, G* __restrict__ gamma, Geo g, uint32_t v0, uint32_t nv,
+                         unsigned char* __restrict__ packed) {
+  const size_t rec = 8 + sizeof(G);
+  const size_t total = (size_t)nv * g.nb;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)(i / g.nb), b = (uint32_t)(i % g.nb);
+    const size_t o = (size_t)(v0 + v) * g.stride + b;
+    if (MODE == 2) {
+      store_packed<G>(packed + i * rec, alpha[o], gamma[o]);
+    } else {
+      uint64_t a;
+      G gm;
+      load_packed<G>(packed + i * rec, &a, &gm);
+      if (MODE == 0) { alpha[o] ^= a; gamma[o] ^= gm; }
+      else { alpha[o] = a; gamma[o] = gm; }
+    }
+  }
+}
+
+}  // namespace
+
+// ================================================================================================
+// launchers
+// ================================================================================================
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+size_t update_tiles_lds_bytes(const Geo& g, int gbytes) {
+  const size_t lds_n = (size_t)g.C * g.Dp + 1;
+  return g.C * 8 + lds_n * 8 + lds_n * (size_t)gbytes;
+}
+
+void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
+                         uint32_t grid_cap, uint32_t* work_counter, const uint32_t* dsts, const uint64_t* raw_idx,
+                         uint64_t* alpha, void* gamma, const Geo& g, const uint64_t* col_bf, hipStream_t s) {
+  constexpr int B = 256;
+  uint32_t grid = grid_cap;
+  if (work_counter) {
+    if (!ntiles_dev) grid = ntiles_host < grid_cap ? ntiles_host : grid_cap;
+    launch_fill_u32(work_counter, 0, 1, s);
+  } else {
+    grid = ntiles_host;  // static mapping: one workgroup per tile
+  }
+  if (grid == 0) return;
+  const size_t lds = update_tiles_lds_bytes(g, gbytes);
+  if (gbytes == 4) {
+    if (raw_idx)
+      hipLaunchKernelGGL((k_update_tiles<uint32_t, true, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
+                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint32_t*)gamma, g, col_bf);
+    else
+      hipLaunchKernelGGL((k_update_tiles<uint32_t, false, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
+                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint32_t*)gamma, g, col_bf);
+  } else {
+    if (raw_idx)
+      hipLaunchKernelGGL((k_update_tiles<uint64_t, true, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
+                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint64_t*)gamma, g, col_bf);
+    else
+      hipLaunchKernelGGL((k_update_tiles<uint64_t, false, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
+                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint64_t*)gamma, g, col_bf);
+  }
+}
+
+size_t group_partials_len(uint32_t n) { return (n + kScanSpan - 1) / kScanSpan + 1; }
+
+void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile, const GroupBufs& b, hipStream_t s) {
+  launch_fill_u32(b.cnt, 0, n, s);
+  const uint2* p2 = reinterpret_cast<const uint2*>(pairs);
+  if (count) hipLaunchKernelGGL(k_count_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cnt, b.bad);
+  scan_counts(b.cnt, n, tile, b.partials, b.off, b.toff, b.cursor, b.totals, s);
+  if (count) hipLaunchKernelGGL(k_scatter_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cursor, b.dst_arr);
+  hipLaunchKernelGGL(k_make_tiles, dim3(grid_for(n, 256)), dim3(256), 0, s, b.cnt, b.off, b.toff, n, tile, b.tiles);
+}
+
+void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d, hipStream_t s) {
+  if (!count) return;
+  const uint2* p2 = reinterpret_cast<const uint2*>(pairs);
+  const uint32_t grid = grid_for(count, 256);
+  hipLaunchKernelGGL(k_eager_union, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
+  hipLaunchKernelGGL(k_eager_sf_insert, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
+  hipLaunchKernelGGL(k_eager_sf_check, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
+}
+
+void launch_dsu_reset(const DsuDev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_dsu_reset, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.parent, d.n);
+  launch_fill_u64(d.sf_hash, kEmpty, d.sf_mask + 1, s);
+  launch_fill_u32(d.flags, 0, kNumFlags, s);
+}
+
+void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
+                          hipStream_t s) {
+  const uint32_t grid = grid_for((size_t)g.n * kWave, 256, 16384);
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, d);
+  else
+    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, d);
+}
+
+void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
+                  const DsuDev& d, const RoundBufs& rb, hipStream_t s) {
+  const uint32_t n = g.n;
+  const uint32_t nslots = n / kWave + 1;
+  hipLaunchKernelGGL(k_compress, dim3(grid_for(n, 256)), dim3(256), 0, s, d.parent, n, rb.root);
+  launch_fill_u32(rb.cnt, 0, n, s);
+  hipLaunchKernelGGL(k_count_roots, dim3(grid_for(n, 256)), dim3(256), 0, s, rb.root, n, rb.cnt);
+  scan_counts(rb.cnt, n, 0, rb.partials, rb.off, nullptr, rb.cursor, rb.totals, s);
+  hipLaunchKernelGGL(k_scatter_roots, dim3(grid_for(n, 256)), dim3(256), 0, s, rb.root, n, rb.cursor, rb.members);
+  launch_fill_u64(rb.acc_a, 0, (size_t)nslots * kWave, s);
+  launch_fill_u64(rb.acc_g, 0, (size_t)nslots * kWave, s);
+  launch_fill_u32(rb.done, 0, nslots, s);
+  launch_fill_u32(d.flags + kFlagModified, 0, 1, s);
+  const uint32_t grid = (uint32_t)(((size_t)((n + kWave - 1) / kWave) * kWave + 255) / 256);
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_reduce_sample<uint32_t>, dim3(grid), dim3(256), 0, s, rb.members, rb.root, rb.off, rb.cnt,
+                       round, alpha, (const uint32_t*)gamma, g, d, rb.acc_a, rb.acc_g, rb.done);
+  else
+    hipLaunchKernelGGL(k_reduce_sample<uint64_t>, dim3(grid), dim3(256), 0, s, rb.members, rb.root, rb.off, rb.cnt,
+                       round, alpha, (const uint64_t*)gamma, g, d, rb.acc_a, rb.acc_g, rb.done);
+}
+
+void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s) {
+  launch_fill_u32(ncc, 0, 1, s);
+  hipLaunchKernelGGL(k_labels, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.parent, d.n, labels, ncc);
+}
+void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,
+                          hipStream_t s) {
+  if (!nq) return;
+  hipLaunchKernelGGL(k_point_queries, dim3(grid_for(nq, 256)), dim3(256), 0, s, d.parent, qa, qb, nq, out);
+}
+
+void launch_sketch_sample(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t sample_idx,
+                          uint32_t* out, hipStream_t s) {
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_sketch_sample<uint32_t>, dim3(1), dim3(kWave), 0, s, alpha, (const uint32_t*)gamma, g,
+                       sample_idx, out);
+  else
+    hipLaunchKernelGGL(k_sketch_sample<uint64_t>, dim3(1), dim3(kWave), 0, s, alpha, (const uint64_t*)gamma, g,
+                       sample_idx, out);
+}
+void launch_xor_range(int gbytes, uint64_t* dst_a, void* dst_g, const uint64_t* src_a, const void* src_g,
+                      const Geo& g, uint32_t nsk, uint32_t b0, uint32_t cnt, bool with_det, hipStream_t s) {
+  const size_t total = (size_t)nsk * (cnt + (with_det ? 1 : 0));
+  if (!total) return;
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_xor_range<uint32_t>, dim3(grid_for(total, 256)), dim3(256), 0, s, dst_a, (uint32_t*)dst_g,
+                       src_a, (const uint32_t*)src_g, g, nsk, b0, cnt, with_det ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_xor_range<uint64_t>, dim3(grid_for(total, 256)), dim3(256), 0, s, dst_a, (uint64_t*)dst_g,
+                       src_a, (const uint64_t*)src_g, g, nsk, b0, cnt, with_det ? 1 : 0);
+}
+
+template <int MODE>
+static void launch_packed(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
+                          const void* packed, hipStream_t s) {
+  const size_t total = (size_t)nv * g.nb;
+  if (!total) return;
+  unsigned char* p = (unsigned char*)packed;
+  if (gbytes == 4)
+    hipLaunchKernelGGL((k_packed<uint32_t, MODE>), dim3(grid_for(total, 256)), dim3(256), 0, s, alpha,
+                       (uint32_t*)gamma, g, v0, nv, p);
+  else
+    hipLaunchKernelGGL((k_packed<uint64_t, MODE>), dim3(grid_for(total, 256)), dim3(256), 0, s, alpha,
+                       (uint64_t*)gamma, g, v0, nv, p);
+}
+void launch_merge_packed(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
+                         const void* packed, hipStream_t s) {
+  launch_packed<0>(gbytes, alpha, gamma, g, v0, nv, packed, s);
+}
+void launch_unpack(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
+                   const void* packed, hipStream_t s) {
+  launch_packed<1>(gbytes, alpha, gamma, g, v0, nv, packed, s);
+}
+void launch_pack(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
+                 void* packed, hipStream_t s) {
+  launch_packed<2>(gbytes, const_cast<uint64_t*>(alpha), const_cast<void*>(gamma), g, v0, nv, packed, s);
+}
+
+}  // namespace gz

@@ -1,0 +1,102 @@
+// gz_kernels.hpp -- host-side launchers for the gfx950 kernels in gz_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gz_common.hpp"
+
+namespace gz {
+
+// Device-side state of one DSU / spanning-forest instance.
+struct DsuDev {
+  uint32_t* parent;    // n
+  uint64_t* sf_hash;   // open-addressing set of spanning-forest edge ids, cap = sf_mask + 1
+  uint64_t sf_mask;
+  uint32_t* sf_list;   // 2 * (n - 1) endpoints, appended in merge order
+  uint32_t* flags;     // FlagSlot entries
+  uint32_t prio_seed;
+  uint32_t n;
+};
+
+enum FlagSlot {
+  kFlagModified = 0,
+  kFlagEagerInvalid = 1,
+  kFlagSfCount = 2,
+  kFlagBad = 3,       // a sample decoded to an out-of-range vertex
+  kFlagBadInput = 4,  // an ingested pair named a vertex >= n (the pair was skipped)
+  kNumFlags = 8
+};
+
+// fills (graph-capture safe replacements for hipMemsetAsync)
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);
+
+// ---- sketch update ----
+// Persistent tile kernel.  `ntiles_dev` (device scalar) bounds the tile index when non-null,
+// otherwise `ntiles_host`.  With `work_counter` the grid is persistent (the launcher clears the
+// counter); with work_counter == nullptr workgroup b processes tile b (grid = ntiles_host).
+// Exactly one of `dsts` (destination vertex ids; index = concat(vertex, dst)) or `raw_idx`
+// (64-bit indices) is non-null.
+void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
+                         uint32_t grid_cap, uint32_t* work_counter, const uint32_t* dsts, const uint64_t* raw_idx,
+                         uint64_t* alpha, void* gamma, const Geo& g, const uint64_t* col_bf, hipStream_t s);
+size_t update_tiles_lds_bytes(const Geo& g, int gbytes);
+
+// ---- grouping of a device-resident stream of (src,dst) pairs into per-vertex tiles ----
+struct GroupBufs {
+  uint32_t* cnt;       // n      per-vertex directed-update counts (zeroed by launch_group)
+  uint32_t* off;       // n + 1  exclusive scan of cnt
+  uint32_t* toff;      // n + 1  exclusive scan of per-vertex tile counts
+  uint32_t* cursor;    // n      scatter cursors
+  uint64_t* partials;  // scan block partials
+  uint32_t* totals;    // [0] total directed updates, [1] total tiles
+  uint32_t* dst_arr;   // 2 * max_pairs
+  Tile* tiles;         // max tiles
+  uint32_t* bad;       // set to 1 when a pair names a vertex >= n
+};
+size_t group_partials_len(uint32_t n);
+// pairs: 2*count u32 (src,dst) in HBM.  Produces tiles of at most `tile` destinations.
+void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile, const GroupBufs& b, hipStream_t s);
+
+// ---- eager DSU (batch form of pre_insert, src/cc_sketch_alg.cpp:79-104) ----
+void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d, hipStream_t s);
+
+// ---- Borůvka ----
+void launch_dsu_reset(const DsuDev& d, hipStream_t s);
+void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
+                          hipStream_t s);
+struct RoundBufs {
+  uint32_t* root;      // n
+  uint32_t* cnt;       // n
+  uint32_t* off;       // n + 1
+  uint32_t* cursor;    // n
+  uint32_t* members;   // n
+  uint64_t* partials;  // scan partials
+  uint32_t* totals;    // scratch for the scan
+  uint64_t* acc_a;     // (n/64 + 1) * 64
+  uint64_t* acc_g;     // (n/64 + 1) * 64 (u32 or u64 entries stored as u64)
+  uint32_t* done;      // n/64 + 1
+};
+void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
+                  const DsuDev& d, const RoundBufs& rb, hipStream_t s);
+void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
+void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,
+                          hipStream_t s);
+
+// ---- standalone sketch ops ----
+// out[0] = code, out[1..2] = idx (lo, hi)
+void launch_sketch_sample(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t sample_idx,
+                          uint32_t* out, hipStream_t s);
+// dst ^= src on buckets [b0, b0 + cnt) of `nsk` consecutive sketches, plus the deterministic bucket
+void launch_xor_range(int gbytes, uint64_t* dst_a, void* dst_g, const uint64_t* src_a, const void* src_g,
+                      const Geo& g, uint32_t nsk, uint32_t b0, uint32_t cnt, bool with_det, hipStream_t s);
+// XOR a packed (reference-format) bucket image of `nv` sketches starting at v0 into the planes
+void launch_merge_packed(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
+                         const void* packed, hipStream_t s);
+// pack `nv` sketches starting at v0 into the reference's byte image (AoS {u64, gamma})
+void launch_pack(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
+                 void* packed, hipStream_t s);
+// unpack (overwrite) a packed image into the planes
+void launch_unpack(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
+                   const void* packed, hipStream_t s);
+
+}  // namespace gz

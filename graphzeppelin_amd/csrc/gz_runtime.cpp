@@ -1,0 +1,924 @@
+// gz_runtime.cpp -- host runtime behind the C ABI (include/gz_hip.h).
+//
+// One GzAlg owns one graph stream's state on the device: the sketch slab of every vertex, the DSU
+// and spanning-forest set, the Borůvka round buffers and the ingest buffers.  All device work of an
+// instance is ordered on its own HIP stream (default priority).
+#include "gz_runtime.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <fstream>
+
+namespace gz {
+
+int device_cu_count() {
+  static int cus = -1;
+  if (cus < 0) {
+    int dev = 0;
+    GZ_HIP(hipGetDevice(&dev));
+    int v = 0;
+    GZ_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+    cus = v > 0 ? v : 1;
+  }
+  return cus;
+}
+
+void require_device() {
+  int cnt = 0;
+  if (hipGetDeviceCount(&cnt) != hipSuccess || cnt < 1)
+    throw GzError(GZ_ERR_DEVICE, "no HIP device is available: the HIP library needs a GPU");
+}
+
+static Geo make_geo(uint32_t nsk, uint64_t vector_len, uint64_t seed, size_t samples, size_t cols) {
+  if (vector_len < 1) throw GzError(GZ_ERR_INVALID_ARG, "vector length must be >= 1");
+  if (samples < 1 || cols < 1) throw GzError(GZ_ERR_INVALID_ARG, "num_samples and cols_per_sample must be >= 1");
+  Geo g{};
+  g.n = nsk;
+  const size_t D = calc_bkt_per_col(vector_len);
+  if (D < 1 || D > 62) throw GzError(GZ_ERR_INVALID_ARG, "bkt_per_col out of the supported range [1, 62]");
+  g.D = (uint32_t)D;
+  g.samples = (uint32_t)samples;
+  g.cps = (uint32_t)cols;
+  g.C = (uint32_t)(samples * cols);
+  g.nb = g.C * g.D + 1;
+  g.stride = (g.nb + 31u) & ~31u;
+  g.Dp = lds_pitch(g.D);
+  g.seed = seed;
+  g.chk_bf = xxh3_bitflip(seed);
+  return g;
+}
+
+static std::vector<uint64_t> column_bitflips(const Geo& g) {
+  std::vector<uint64_t> bf(g.C);
+  for (uint32_t c = 0; c < g.C; ++c) bf[c] = xxh3_bitflip(g.seed + (uint64_t)c * 5);  // column_seed, sketch.h:175
+  return bf;
+}
+
+static void check_gamma(int gb) {
+  if (gb != 4 && gb != 8) throw GzError(GZ_ERR_INVALID_ARG, "gamma_bytes must be 4 or 8");
+}
+
+// ================================================================================================
+// GzSketch
+// ================================================================================================
+GzSketch::GzSketch(uint64_t vector_len, uint64_t seed, size_t samples, size_t cols, int gamma_bytes) : gb(gamma_bytes) {
+  check_gamma(gb);
+  require_device();
+  geo = make_geo(1, vector_len, seed, samples, cols);
+  if (update_tiles_lds_bytes(geo, gb) > 64 * 1024) throw GzError(GZ_ERR_INVALID_ARG, "sketch too large for one workgroup's LDS");
+  GZ_HIP(hipStreamCreate(&st));
+  alpha.alloc(geo.stride);
+  gamma.alloc((size_t)geo.stride * gb);
+  out.alloc(4);
+  auto bf = column_bitflips(geo);
+  col_bf.alloc(bf.size());
+  GZ_HIP(hipMemcpyAsync(col_bf.p, bf.data(), bf.size() * 8, hipMemcpyHostToDevice, st));
+  zero();
+}
+GzSketch::~GzSketch() {
+  if (st) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+}
+void GzSketch::zero() {  // src/sketch.cpp:86-92
+  launch_fill_u64(alpha.p, 0, geo.stride, st);
+  launch_fill_u64(reinterpret_cast<uint64_t*>(gamma.p), 0, (size_t)geo.stride * gb / 8, st);
+  GZ_HIP(hipStreamSynchronize(st));
+  sample_idx = 0;
+}
+void GzSketch::update_batch(const uint64_t* idx, size_t count) {  // src/sketch.cpp:69-83
+  if (!count) return;
+  constexpr size_t kTile = 1024;
+  raw.ensure(count);
+  const size_t nt = (count + kTile - 1) / kTile;
+  std::vector<Tile> ht(nt);
+  for (size_t t = 0; t < nt; ++t)
+    ht[t] = Tile{0, (uint32_t)(t * kTile), (uint32_t)std::min(kTile, count - t * kTile), nt > 1 ? 1u : 0u};
+  tiles.ensure(nt);
+  GZ_HIP(hipMemcpyAsync(raw.p, idx, count * 8, hipMemcpyHostToDevice, st));
+  GZ_HIP(hipMemcpyAsync(tiles.p, ht.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
+  launch_update_tiles(gb, tiles.p, nullptr, (uint32_t)nt, 0, nullptr, nullptr, raw.p, alpha.p, gamma.p, geo, col_bf.p, st);
+  GZ_HIP(hipGetLastError());
+  GZ_HIP(hipStreamSynchronize(st));
+}
+int GzSketch::sample(uint64_t* idx) {  // src/sketch.cpp:94-116
+  if (sample_idx >= geo.samples) throw GzError(GZ_ERR_OUT_OF_SAMPLES, "sketch cannot be sampled more times");
+  const uint32_t s = (uint32_t)sample_idx++;
+  launch_sketch_sample(gb, alpha.p, gamma.p, geo, s, out.p, st);
+  uint32_t h[3];
+  GZ_HIP(hipMemcpyAsync(h, out.p, 12, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+  *idx = ((uint64_t)h[2] << 32) | h[1];
+  return (int)h[0];
+}
+void GzSketch::merge(const GzSketch& o) {  // src/sketch.cpp:149-154
+  if (o.geo.nb != geo.nb || o.gb != gb) throw GzError(GZ_ERR_INVALID_ARG, "sketch shapes differ");
+  GZ_HIP(hipStreamSynchronize(o.st));
+  launch_xor_range(gb, alpha.p, gamma.p, o.alpha.p, o.gamma.p, geo, 1, 0, geo.nb - 1, true, st);
+  GZ_HIP(hipStreamSynchronize(st));
+}
+void GzSketch::range_merge(const GzSketch& o, size_t start, size_t ns) {  // src/sketch.cpp:156-179
+  if (o.geo.nb != geo.nb || o.gb != gb) throw GzError(GZ_ERR_INVALID_ARG, "sketch shapes differ");
+  if (start + ns > geo.samples) {
+    sample_idx = geo.samples;  // the reference leaves the sketch in a fail state
+    return;
+  }
+  sample_idx = std::max(sample_idx, start);
+  GZ_HIP(hipStreamSynchronize(o.st));
+  const uint32_t b0 = (uint32_t)(start * geo.cps * geo.D), cnt = (uint32_t)(ns * geo.cps * geo.D);
+  launch_xor_range(gb, alpha.p, gamma.p, o.alpha.p, o.gamma.p, geo, 1, b0, cnt, true, st);
+  GZ_HIP(hipStreamSynchronize(st));
+}
+void GzSketch::merge_raw(const void* rawb) {  // src/sketch.cpp:181-186
+  packed.ensure(bytes());
+  GZ_HIP(hipMemcpyAsync(packed.p, rawb, bytes(), hipMemcpyHostToDevice, st));
+  launch_merge_packed(gb, alpha.p, gamma.p, geo, 0, 1, packed.p, st);
+  GZ_HIP(hipStreamSynchronize(st));
+}
+void GzSketch::serialize(void* o, size_t nbytes) const {  // src/sketch.cpp:188-190
+  if (nbytes < bytes()) throw GzError(GZ_ERR_INVALID_ARG, "output buffer smaller than bucket_array_bytes()");
+  auto* self = const_cast<GzSketch*>(this);
+  self->packed.ensure(bytes());
+  launch_pack(gb, alpha.p, gamma.p, geo, 0, 1, self->packed.p, st);
+  GZ_HIP(hipMemcpyAsync(o, packed.p, bytes(), hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+}
+void GzSketch::deserialize(const void* in, size_t nbytes) {  // src/sketch.cpp:23-35
+  if (nbytes < bytes()) throw GzError(GZ_ERR_INVALID_ARG, "input buffer smaller than bucket_array_bytes()");
+  packed.ensure(bytes());
+  GZ_HIP(hipMemcpyAsync(packed.p, in, bytes(), hipMemcpyHostToDevice, st));
+  launch_unpack(gb, alpha.p, gamma.p, geo, 0, 1, packed.p, st);
+  GZ_HIP(hipStreamSynchronize(st));
+}
+
+// ================================================================================================
+// GzAlg
+// ================================================================================================
+static size_t default_cap_pairs(uint32_t n, size_t desired) {
+  const size_t want = (size_t)n * desired / 2;
+  return std::min<size_t>(size_t(1) << 26, std::max<size_t>(size_t(1) << 20, want));
+}
+
+uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
+  const uint64_t len = calc_vector_length(n);
+  const size_t D = calc_bkt_per_col(len), C = calc_cc_samples(n, c.sketches_factor);
+  const uint64_t nb = C * D + 1, stride = (nb + 31) & ~31ull;
+  const int gbytes = c.gamma_bytes ? c.gamma_bytes : 4;
+  const uint64_t slab = (uint64_t)n * stride * (8 + gbytes);
+  const size_t desired = (size_t)(nb * (8 + gbytes) / 4 * c.batch_factor);
+  const uint64_t cap = c.flush_threshold ? c.flush_threshold : default_cap_pairs(n, desired);
+  const uint32_t t = c.tile ? c.tile : 1024;
+  const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ +
+                          (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
+  const uint64_t rounds = (uint64_t)n * 40 + (uint64_t)n * 16 /*sf hash*/;
+  return slab + ingest + rounds;
+}
+
+GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(seed_), cfg(c) {
+  if (n < 2) throw GzError(GZ_ERR_INVALID_ARG, "num_vertices must be >= 2");
+  if (n > (1u << 31)) throw GzError(GZ_ERR_INVALID_ARG, "num_vertices above 2^31 is not supported");
+  if (cfg.sketches_factor <= 0) cfg.sketches_factor = 1;  // cc_alg_configuration.cpp:15-23
+  if (cfg.batch_factor <= 0) cfg.batch_factor = 1;        // cc_alg_configuration.cpp:25-33
+  if (cfg.gamma_bytes == 0) cfg.gamma_bytes = 4;
+  gb = cfg.gamma_bytes;
+  check_gamma(gb);
+  require_device();
+  geo = make_geo(n, calc_vector_length(n), seed, calc_cc_samples(n, cfg.sketches_factor), 1);
+  if (update_tiles_lds_bytes(geo, gb) > 64 * 1024) throw GzError(GZ_ERR_INVALID_ARG, "sketch too large for LDS");
+  tile = cfg.tile ? cfg.tile : 1024;
+  const size_t lds = update_tiles_lds_bytes(geo, gb);
+  const size_t per_cu = std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
+  grid_cap = (uint32_t)(device_cu_count() * per_cu);
+  GZ_HIP(hipStreamCreate(&st));
+
+  // sketch slab
+  alpha.alloc((size_t)n * geo.stride);
+  gamma.alloc((size_t)n * geo.stride * gb);
+  launch_fill_u64(alpha.p, 0, alpha.n, st);
+  launch_fill_u64(reinterpret_cast<uint64_t*>(gamma.p), 0, gamma.n / 8, st);
+  auto bf = column_bitflips(geo);
+  col_bf.alloc(bf.size());
+  GZ_HIP(hipMemcpyAsync(col_bf.p, bf.data(), bf.size() * 8, hipMemcpyHostToDevice, st));
+
+  // DSU + forest
+  parent.alloc(n);
+  uint64_t cap = 1;
+  while (cap < 2ull * n) cap <<= 1;
+  sf_hash.alloc(cap);
+  sf_list.alloc(2ull * n);
+  flags.alloc(kNumFlags);
+  labels.alloc(n + 1);
+  dsu.parent = parent.p;
+  dsu.sf_hash = sf_hash.p;
+  dsu.sf_mask = cap - 1;
+  dsu.sf_list = sf_list.p;
+  dsu.flags = flags.p;
+  dsu.prio_seed = (uint32_t)(seed * 0x9E3779B97F4A7C15ULL >> 32);
+  dsu.n = n;
+  launch_dsu_reset(dsu, st);
+
+  // Borůvka round buffers
+  const uint32_t nslots = n / 64 + 1;
+  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n + 1); r_cursor.alloc(n); r_members.alloc(n);
+  r_totals.alloc(4); r_done.alloc(nslots);
+  r_partials.alloc(group_partials_len(n));
+  r_acc_a.alloc((size_t)nslots * 64); r_acc_g.alloc((size_t)nslots * 64);
+  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, r_members.p, r_partials.p, r_totals.p,
+                 r_acc_a.p, r_acc_g.p, r_done.p};
+
+  // device ingest
+  cap_pairs = cfg.flush_threshold ? (size_t)cfg.flush_threshold : default_cap_pairs(n, desired_updates_per_batch());
+  if (cap_pairs > (size_t(1) << 30)) cap_pairs = size_t(1) << 30;
+  pending.alloc(2 * cap_pairs);
+  dst_arr.alloc(2 * cap_pairs);
+  merged.alloc(cap_pairs);
+  g_cnt.alloc(n); g_off.alloc(n + 1); g_toff.alloc(n + 1); g_cursor.alloc(n); g_totals.alloc(4);
+  g_partials.alloc(group_partials_len(n));
+  g_tiles.alloc(2 * cap_pairs / tile + n + 1);
+  work_counter.alloc(1);
+  gbufs = GroupBufs{g_cnt.p, g_off.p, g_toff.p, g_cursor.p, g_partials.p, g_totals.p, dst_arr.p, g_tiles.p,
+                    flags.p + kFlagBadInput};
+
+  // host staging
+  stage_cap = std::max<size_t>(desired_updates_per_batch(), cfg.batched ? (size_t(1) << 22) : 0);
+  stages.resize(cfg.batched ? 2 : 4);
+  vstamp.assign(n, 0);
+  vtiles.assign(n, 0);
+  eager_cap = size_t(1) << 20;
+  hflags.alloc(kNumFlags);
+  hsmall.alloc(16);
+  GZ_HIP(hipEventCreateWithFlags(&eager_ev, hipEventDisableTiming));
+  GZ_HIP(hipStreamSynchronize(st));
+}
+
+GzAlg::~GzAlg() {
+  if (st) (void)hipStreamSynchronize(st);
+  for (auto& s : stages)
+    if (s.ev) (void)hipEventDestroy(s.ev);
+  if (eager_ev) (void)hipEventDestroy(eager_ev);
+  if (st) (void)hipStreamDestroy(st);
+}
+
+// ---- ingest --------------------------------------------------------------------------------------
+
+// pre_insert, src/cc_sketch_alg.cpp:79-104: buffered and applied on the device in batches.
+void GzAlg::pre_insert(uint32_t src, uint32_t dst) {
+  if (src >= n || dst >= n) throw GzError(GZ_ERR_INVALID_ARG, "vertex id out of range");
+  if (!cfg.eager_dsu) {
+    if (dsu_valid) dsu_valid = false;
+    return;
+  }
+  if (!dsu_valid) return;
+  std::lock_guard<std::mutex> lk(eager_mtx);
+  if (!eager_host.p) {
+    eager_host.alloc(2 * eager_cap);
+    eager_dev.alloc(2 * eager_cap);
+    eager_merged.alloc(eager_cap);
+  }
+  if (eager_used == eager_cap) submit_eager_locked();
+  eager_host.p[2 * eager_used] = src;
+  eager_host.p[2 * eager_used + 1] = dst;
+  ++eager_used;
+}
+
+void GzAlg::submit_eager_locked() {
+  if (!eager_used) return;
+  GZ_HIP(hipMemcpyAsync(eager_dev.p, eager_host.p, eager_used * 8, hipMemcpyHostToDevice, st));
+  launch_eager(eager_dev.p, eager_used, eager_merged.p, dsu, st);
+  GZ_HIP(hipGetLastError());
+  GZ_HIP(hipEventRecord(eager_ev, st));
+  GZ_HIP(hipEventSynchronize(eager_ev));  // the host buffer is reused right away
+  eager_used = 0;
+}
+
+// apply_update_batch, src/cc_sketch_alg.cpp:106-118
+void GzAlg::apply_update_batch(uint32_t src, const uint32_t* dsts, size_t k) {
+  if (update_locked) throw GzError(GZ_ERR_UPDATE_LOCKED, "Cannot update the algorithm: Connected components currently running");
+  if (src >= n) throw GzError(GZ_ERR_INVALID_ARG, "vertex id out of range");
+  if (!k) return;
+  if (k > 0xffffffffull) throw GzError(GZ_ERR_INVALID_ARG, "batch too large");
+  std::lock_guard<std::mutex> lk(stage_mtx);
+  if (!cfg.batched) {
+    plain_launch(src, dsts, k);
+    return;
+  }
+  Stage* s = &stages[cur_stage];
+  if (s->used + k > s->dcap || s->ent_v.size() >= kMaxEntries) {
+    submit_stage_locked();
+    s = &stages[cur_stage];
+  }
+  if (s->inflight) {
+    GZ_HIP(hipEventSynchronize(s->ev));
+    s->inflight = false;
+  }
+  if (s->dcap < k) {  // first use, or a batch larger than the stage: (re)allocate an empty stage
+    const size_t dcap = std::max(stage_cap, k);
+    s->max_tiles = dcap / tile + kMaxEntries + 1;
+    s->host.alloc(s->max_tiles * sizeof(Tile) + dcap * 4);
+    s->dev.alloc(s->host.n);
+    s->dcap = dcap;
+  }
+  uint32_t* hd = reinterpret_cast<uint32_t*>(s->host.p + s->max_tiles * sizeof(Tile));
+  std::memcpy(hd + s->used, dsts, k * 4);
+  s->ent_v.push_back(src);
+  s->ent_off.push_back((uint32_t)s->used);
+  s->ent_cnt.push_back((uint32_t)k);
+  s->used += k;
+  total_updates += k;
+}
+
+// Submit the current batched stage: build tiles (split flag when a vertex has several tiles in
+// this launch), copy tiles and destinations to the device and launch one update kernel over all.
+// Host layout of a stage: [max_tiles Tile records | dcap destination ids].
+void GzAlg::submit_stage_locked() {
+  Stage& s = stages[cur_stage];
+  if (!s.used) return;
+  ++epoch;
+  if (epoch == 0) {
+    std::fill(vstamp.begin(), vstamp.end(), 0u);
+    epoch = 1;
+  }
+  for (size_t e = 0; e < s.ent_v.size(); ++e) {
+    const uint32_t v = s.ent_v[e];
+    if (vstamp[v] != epoch) {
+      vstamp[v] = epoch;
+      vtiles[v] = 0;
+    }
+    vtiles[v] += (s.ent_cnt[e] + tile - 1) / tile;
+  }
+  Tile* ht = reinterpret_cast<Tile*>(s.host.p);
+  size_t nt = 0;
+  for (size_t e = 0; e < s.ent_v.size(); ++e) {
+    const uint32_t v = s.ent_v[e], c = s.ent_cnt[e], o = s.ent_off[e];
+    const uint32_t split = vtiles[v] > 1 ? 1u : 0u;
+    for (uint32_t b = 0; b < c; b += tile) ht[nt++] = Tile{v, o + b, std::min(tile, c - b), split};
+  }
+  const size_t doff = s.max_tiles * sizeof(Tile);
+  GZ_HIP(hipMemcpyAsync(s.dev.p, s.host.p, nt * sizeof(Tile), hipMemcpyHostToDevice, st));
+  GZ_HIP(hipMemcpyAsync(s.dev.p + doff, s.host.p + doff, s.used * 4, hipMemcpyHostToDevice, st));
+  launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
+                      reinterpret_cast<uint32_t*>(s.dev.p + doff), nullptr, alpha.p, gamma.p, geo, col_bf.p, st);
+  GZ_HIP(hipGetLastError());
+  if (!s.ev) GZ_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+  GZ_HIP(hipEventRecord(s.ev, st));
+  s.inflight = true;
+  s.used = 0;
+  s.ent_v.clear();
+  s.ent_off.clear();
+  s.ent_cnt.clear();
+  cur_stage = (cur_stage + 1) % stages.size();
+}
+
+// Plain form: one launch per apply_update_batch call (one gutter flush).
+void GzAlg::plain_launch(uint32_t src, const uint32_t* dsts, size_t k) {
+  Stage& s = stages[cur_stage];
+  cur_stage = (cur_stage + 1) % stages.size();
+  if (s.inflight) {
+    GZ_HIP(hipEventSynchronize(s.ev));
+    s.inflight = false;
+  }
+  const size_t nt = (k + tile - 1) / tile;
+  const size_t tbytes = ((nt * sizeof(Tile)) + 255) & ~size_t(255);
+  const size_t need = tbytes + k * 4;
+  if (s.host.n < need) {
+    s.host.alloc(need);
+    s.dev.alloc(need);
+  }
+  Tile* ht = reinterpret_cast<Tile*>(s.host.p);
+  for (size_t t = 0; t < nt; ++t)
+    ht[t] = Tile{src, (uint32_t)(t * tile), (uint32_t)std::min<size_t>(tile, k - t * tile), nt > 1 ? 1u : 0u};
+  std::memcpy(s.host.p + tbytes, dsts, k * 4);
+  GZ_HIP(hipMemcpyAsync(s.dev.p, s.host.p, need, hipMemcpyHostToDevice, st));
+  launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
+                      reinterpret_cast<uint32_t*>(s.dev.p + tbytes), nullptr, alpha.p, gamma.p, geo, col_bf.p, st);
+  GZ_HIP(hipGetLastError());
+  if (!s.ev) GZ_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+  GZ_HIP(hipEventRecord(s.ev, st));
+  s.inflight = true;
+  total_updates += k;
+}
+
+// apply_raw_buckets_update, src/cc_sketch_alg.cpp:120-123
+void GzAlg::apply_raw(uint32_t src, const void* rawb) {
+  if (src >= n) throw GzError(GZ_ERR_INVALID_ARG, "vertex id out of range");
+  std::lock_guard<std::mutex> lk(stage_mtx);
+  DevBuf<uint8_t> tmp;
+  tmp.alloc(sketch_bytes());
+  GZ_HIP(hipMemcpyAsync(tmp.p, rawb, sketch_bytes(), hipMemcpyHostToDevice, st));
+  launch_merge_packed(gb, alpha.p, gamma.p, geo, src, 1, tmp.p, st);
+  GZ_HIP(hipStreamSynchronize(st));
+}
+
+void GzAlg::flush_pending() {
+  if (!pending_n) return;
+  launch_group(pending.p, pending_n, n, tile, gbufs, st);
+  launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, work_counter.p, dst_arr.p, nullptr, alpha.p, gamma.p,
+                      geo, col_bf.p, st);
+  GZ_HIP(hipGetLastError());
+  total_updates += 2 * pending_n;
+  pending_n = 0;
+}
+
+void GzAlg::ingest_device(const uint32_t* pairs_dev, size_t count) {
+  if (update_locked) throw GzError(GZ_ERR_UPDATE_LOCKED, "Cannot update the algorithm: Connected components currently running");
+  if (!count) return;
+  std::lock_guard<std::mutex> lk(stage_mtx);
+  if (!cfg.eager_dsu) dsu_valid = false;
+  while (count) {
+    const size_t take = std::min(count, cap_pairs - pending_n);
+    uint32_t* dst = pending.p + 2 * pending_n;
+    GZ_HIP(hipMemcpyAsync(dst, pairs_dev, take * 8, hipMemcpyDeviceToDevice, st));
+    if (cfg.eager_dsu && dsu_valid) launch_eager(dst, take, merged.p, dsu, st);
+    pending_n += take;
+    pairs_dev += 2 * take;
+    count -= take;
+    if (pending_n == cap_pairs) flush_pending();
+  }
+}
+
+void GzAlg::ingest_host(const uint32_t* pairs, size_t count) {
+  if (update_locked) throw GzError(GZ_ERR_UPDATE_LOCKED, "Cannot update the algorithm: Connected components currently running");
+  if (!count) return;
+  std::lock_guard<std::mutex> lk(stage_mtx);
+  if (!cfg.eager_dsu) dsu_valid = false;
+  while (count) {
+    const size_t take = std::min(count, cap_pairs - pending_n);
+    uint32_t* dst = pending.p + 2 * pending_n;
+    GZ_HIP(hipMemcpyAsync(dst, pairs, take * 8, hipMemcpyHostToDevice, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (cfg.eager_dsu && dsu_valid) launch_eager(dst, take, merged.p, dsu, st);
+    pending_n += take;
+    pairs += 2 * take;
+    count -= take;
+    if (pending_n == cap_pairs) flush_pending();
+  }
+}
+
+// The driver's force_flush + flush_workers (include/graph_sketch_driver.h:193-203)
+void GzAlg::flush_all() {
+  {
+    std::lock_guard<std::mutex> lk(stage_mtx);
+    if (cfg.batched) submit_stage_locked();
+    flush_pending();
+  }
+  if (read_flag(kFlagBadInput)) {
+    launch_fill_u32(flags.p + kFlagBadInput, 0, 1, st);
+    GZ_HIP(hipStreamSynchronize(st));
+    throw GzError(GZ_ERR_INVALID_ARG, "ingested device pairs named a vertex >= num_vertices; those pairs were skipped");
+  }
+}
+
+uint32_t GzAlg::read_flag(int slot) {
+  GZ_HIP(hipMemcpyAsync(hflags.p, flags.p, kNumFlags * 4, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+  return hflags.p[slot];
+}
+
+void GzAlg::sync_eager_state() {
+  if (!cfg.eager_dsu) return;
+  {
+    std::lock_guard<std::mutex> lk(eager_mtx);
+    if (dsu_valid) submit_eager_locked();
+    else eager_used = 0;
+  }
+  if (dsu_valid && read_flag(kFlagEagerInvalid)) dsu_valid = false;
+}
+
+// has_cached_query, include/cc_sketch_alg.h:184-190
+bool GzAlg::has_cached_query(int code) {
+  if (code != GZ_CONNECTIVITY) return false;
+  sync_eager_state();
+  return dsu_valid;
+}
+
+void GzAlg::prep_query(int code) {
+  if (has_cached_query(code)) return;
+  flush_all();
+}
+
+void GzAlg::ensure_answer() {
+  sync_eager_state();
+  if (dsu_valid) return;
+  flush_all();
+  boruvka();
+}
+
+// boruvka_emulation, src/cc_sketch_alg.cpp:464-513
+void GzAlg::boruvka() {
+  update_locked = true;
+  try {
+    launch_dsu_reset(dsu, st);
+    launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, st);
+    GZ_HIP(hipGetLastError());
+    uint32_t round = 0;
+    bool modified = read_flag(kFlagModified) != 0;
+    while (modified) {
+      ++round;
+      if (round >= geo.samples)
+        throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
+      launch_round(gb, round, alpha.p, gamma.p, geo, dsu, rb, st);
+      GZ_HIP(hipGetLastError());
+      modified = read_flag(kFlagModified) != 0;
+    }
+    last_rounds = round;
+    dsu_valid = true;
+  } catch (...) {
+    update_locked = false;
+    dsu_valid = false;
+    throw;
+  }
+  update_locked = false;
+}
+
+// connected_components, src/cc_sketch_alg.cpp:515-557 + ConnectedComponents, return_types.cpp:5-19
+uint32_t GzAlg::connected_components(uint32_t* parent_out) {
+  std::lock_guard<std::mutex> lk(query_mtx);
+  ensure_answer();
+  launch_labels(dsu, labels.p, labels.p + n, st);
+  uint32_t ncc = 0;
+  if (parent_out) GZ_HIP(hipMemcpyAsync(parent_out, labels.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipMemcpyAsync(hsmall.p, labels.p + n, 4, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+  ncc = hsmall.p[0];
+  return ncc;
+}
+
+// point_query, src/cc_sketch_alg.cpp:570-612 (many at once, one consistent state)
+void GzAlg::point_queries(const uint32_t* us, const uint32_t* vs, size_t count, uint8_t* outp) {
+  for (size_t i = 0; i < count; ++i)
+    if (us[i] >= n || vs[i] >= n) throw GzError(GZ_ERR_INVALID_ARG, "vertex id out of range");
+  std::lock_guard<std::mutex> lk(query_mtx);
+  ensure_answer();
+  if (!count) return;
+  qa.ensure(count);
+  qb.ensure(count);
+  qout.ensure(count);
+  GZ_HIP(hipMemcpyAsync(qa.p, us, count * 4, hipMemcpyHostToDevice, st));
+  GZ_HIP(hipMemcpyAsync(qb.p, vs, count * 4, hipMemcpyHostToDevice, st));
+  launch_point_queries(dsu, qa.p, qb.p, (uint32_t)count, qout.p, st);
+  GZ_HIP(hipMemcpyAsync(outp, qout.p, count, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+}
+
+// calc_spanning_forest, src/cc_sketch_alg.cpp:559-568
+size_t GzAlg::spanning_forest(uint32_t* edges, size_t cap) {
+  connected_components(nullptr);
+  const uint32_t k = std::min<uint32_t>(read_flag(kFlagSfCount), n);
+  if (edges && cap) {
+    const size_t c = std::min<size_t>(cap, k);
+    GZ_HIP(hipMemcpyAsync(edges, sf_list.p, c * 8, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+  }
+  return k;
+}
+
+void GzAlg::dump(uint32_t v0, uint32_t count, void* outp) {
+  if ((uint64_t)v0 + count > n) throw GzError(GZ_ERR_INVALID_ARG, "vertex range out of bounds");
+  flush_all();
+  const size_t sb = sketch_bytes();
+  const uint32_t chunk = (uint32_t)std::max<size_t>(1, (size_t(64) << 20) / sb);
+  DevBuf<uint8_t> tmp;
+  tmp.alloc((size_t)std::min(chunk, count ? count : 1) * sb);
+  for (uint32_t i = 0; i < count; i += chunk) {
+    const uint32_t c = std::min(chunk, count - i);
+    launch_pack(gb, alpha.p, gamma.p, geo, v0 + i, c, tmp.p, st);
+    GZ_HIP(hipMemcpyAsync((uint8_t*)outp + (size_t)i * sb, tmp.p, (size_t)c * sb, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+  }
+}
+
+void GzAlg::load_packed(uint32_t v0, uint32_t count, const void* in) {
+  const size_t sb = sketch_bytes();
+  const uint32_t chunk = (uint32_t)std::max<size_t>(1, (size_t(64) << 20) / sb);
+  DevBuf<uint8_t> tmp;
+  tmp.alloc((size_t)std::min(chunk, count ? count : 1) * sb);
+  for (uint32_t i = 0; i < count; i += chunk) {
+    const uint32_t c = std::min(chunk, count - i);
+    GZ_HIP(hipMemcpyAsync(tmp.p, (const uint8_t*)in + (size_t)i * sb, (size_t)c * sb, hipMemcpyHostToDevice, st));
+    launch_unpack(gb, alpha.p, gamma.p, geo, v0 + i, c, tmp.p, st);
+    GZ_HIP(hipStreamSynchronize(st));
+  }
+}
+
+}  // namespace gz
+
+// ==================================================================================================
+// C ABI
+// ==================================================================================================
+using gz::GzAlg;
+using gz::GzError;
+using gz::GzSketch;
+
+struct gz_sketch {
+  GzSketch* s;
+};
+struct gz_alg {
+  GzAlg* a;
+};
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return GZ_OK;
+  } catch (const GzError& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return GZ_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return GZ_ERR_DEVICE;
+  }
+}
+
+#define GZ_REQUIRE(cond, msg)                                  \
+  do {                                                         \
+    if (!(cond)) throw GzError(GZ_ERR_INVALID_ARG, msg);       \
+  } while (0)
+
+extern "C" {
+
+const char* gz_last_error(void) { return g_last_error.c_str(); }
+
+int gz_device_count(int* out) {
+  return guarded([&] {
+    GZ_REQUIRE(out, "null out");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *out = c;
+  });
+}
+
+uint64_t gz_calc_vector_length(uint32_t n) { return gz::calc_vector_length(n); }
+size_t gz_calc_cc_samples(uint32_t n, double f) { return gz::calc_cc_samples(n, f); }
+size_t gz_calc_bkt_per_col(uint64_t len) { return gz::calc_bkt_per_col(len); }
+
+// ---- sketch ----
+int gz_sketch_create(uint64_t vector_len, uint64_t seed, size_t num_samples, size_t cols, int gamma_bytes,
+                     gz_sketch** out) {
+  return guarded([&] {
+    GZ_REQUIRE(out, "null out");
+    *out = nullptr;
+    auto* s = new GzSketch(vector_len, seed, num_samples, cols, gamma_bytes);
+    *out = new gz_sketch{s};
+  });
+}
+int gz_sketch_destroy(gz_sketch* s) {
+  return guarded([&] {
+    if (!s) return;
+    delete s->s;
+    delete s;
+  });
+}
+int gz_sketch_update(gz_sketch* s, uint64_t idx) {
+  return guarded([&] { GZ_REQUIRE(s, "null sketch"); s->s->update_batch(&idx, 1); });
+}
+int gz_sketch_update_batch(gz_sketch* s, const uint64_t* idx, size_t count) {
+  return guarded([&] {
+    GZ_REQUIRE(s && (idx || !count), "null argument");
+    s->s->update_batch(idx, count);
+  });
+}
+int gz_sketch_sample(gz_sketch* s, uint64_t* idx, int* result) {
+  return guarded([&] {
+    GZ_REQUIRE(s && idx && result, "null argument");
+    *result = s->s->sample(idx);
+  });
+}
+int gz_sketch_merge(gz_sketch* dst, const gz_sketch* src) {
+  return guarded([&] { GZ_REQUIRE(dst && src, "null sketch"); dst->s->merge(*src->s); });
+}
+int gz_sketch_range_merge(gz_sketch* dst, const gz_sketch* src, size_t start, size_t ns) {
+  return guarded([&] { GZ_REQUIRE(dst && src, "null sketch"); dst->s->range_merge(*src->s, start, ns); });
+}
+int gz_sketch_merge_raw_bucket_buffer(gz_sketch* s, const void* raw) {
+  return guarded([&] { GZ_REQUIRE(s && raw, "null argument"); s->s->merge_raw(raw); });
+}
+int gz_sketch_zero_contents(gz_sketch* s) {
+  return guarded([&] { GZ_REQUIRE(s, "null sketch"); s->s->zero(); });
+}
+int gz_sketch_reset_sample_state(gz_sketch* s) {
+  return guarded([&] { GZ_REQUIRE(s, "null sketch"); s->s->sample_idx = 0; });
+}
+int gz_sketch_serialize(const gz_sketch* s, void* out, size_t bytes) {
+  return guarded([&] { GZ_REQUIRE(s && out, "null argument"); s->s->serialize(out, bytes); });
+}
+int gz_sketch_deserialize(gz_sketch* s, const void* in, size_t bytes) {
+  return guarded([&] { GZ_REQUIRE(s && in, "null argument"); s->s->deserialize(in, bytes); });
+}
+int gz_sketch_equal(const gz_sketch* a, const gz_sketch* b, int* eq) {
+  return guarded([&] {
+    GZ_REQUIRE(a && b && eq, "null argument");
+    if (a->s->geo.nb != b->s->geo.nb || a->s->geo.seed != b->s->geo.seed || a->s->gb != b->s->gb) {
+      *eq = 0;
+      return;
+    }
+    std::vector<uint8_t> x(a->s->bytes()), y(b->s->bytes());
+    a->s->serialize(x.data(), x.size());
+    b->s->serialize(y.data(), y.size());
+    *eq = x == y ? 1 : 0;
+  });
+}
+size_t gz_sketch_bucket_array_bytes(const gz_sketch* s) { return s ? s->s->bytes() : 0; }
+size_t gz_sketch_get_buckets(const gz_sketch* s) { return s ? s->s->geo.nb : 0; }
+size_t gz_sketch_get_columns(const gz_sketch* s) { return s ? s->s->geo.C : 0; }
+size_t gz_sketch_get_num_samples(const gz_sketch* s) { return s ? s->s->geo.samples : 0; }
+size_t gz_sketch_get_bkt_per_col(const gz_sketch* s) { return s ? s->s->geo.D : 0; }
+uint64_t gz_sketch_get_seed(const gz_sketch* s) { return s ? s->s->geo.seed : 0; }
+
+// ---- algorithm ----
+void gz_alg_config_default(gz_alg_config* c) {
+  if (!c) return;
+  c->sketches_factor = 1;
+  c->batch_factor = 1;
+  c->gamma_bytes = 4;
+  c->eager_dsu = 1;
+  c->batched = 1;
+  c->tile = 0;
+  c->flush_threshold = 0;
+}
+int gz_alg_create(uint32_t n, uint64_t seed, const gz_alg_config* cfg, gz_alg** out) {
+  return guarded([&] {
+    GZ_REQUIRE(out, "null out");
+    *out = nullptr;
+    gz_alg_config c;
+    gz_alg_config_default(&c);
+    if (cfg) c = *cfg;
+    auto* a = new GzAlg(n, seed, c);
+    *out = new gz_alg{a};
+  });
+}
+int gz_alg_destroy(gz_alg* a) {
+  return guarded([&] {
+    if (!a) return;
+    delete a->a;
+    delete a;
+  });
+}
+int gz_alg_allocate_worker_memory(gz_alg* a, size_t) {
+  // Worker scratch is the device staging, sized at construction; nothing per worker is needed.
+  return guarded([&] { GZ_REQUIRE(a, "null alg"); });
+}
+size_t gz_alg_get_desired_updates_per_batch(const gz_alg* a) { return a ? a->a->desired_updates_per_batch() : 0; }
+uint32_t gz_alg_get_num_vertices(const gz_alg* a) { return a ? a->a->n : 0; }
+uint64_t gz_alg_get_seed(const gz_alg* a) { return a ? a->a->seed : 0; }
+size_t gz_alg_max_rounds(const gz_alg* a) { return a ? a->a->geo.samples : 0; }
+size_t gz_alg_sketch_bytes(const gz_alg* a) { return a ? a->a->sketch_bytes() : 0; }
+int gz_alg_pre_insert(gz_alg* a, uint32_t src, uint32_t dst, int, int) {
+  return guarded([&] { GZ_REQUIRE(a, "null alg"); a->a->pre_insert(src, dst); });
+}
+int gz_alg_apply_update_batch(gz_alg* a, int, uint32_t src, const uint32_t* dsts, size_t count) {
+  return guarded([&] {
+    GZ_REQUIRE(a && (dsts || !count), "null argument");
+    for (size_t i = 0; i < count; ++i)
+      if (dsts[i] >= a->a->n) throw GzError(GZ_ERR_INVALID_ARG, "destination vertex out of range");
+    a->a->apply_update_batch(src, dsts, count);
+  });
+}
+int gz_alg_apply_raw_buckets_update(gz_alg* a, uint32_t src, const void* raw) {
+  return guarded([&] { GZ_REQUIRE(a && raw, "null argument"); a->a->apply_raw(src, raw); });
+}
+int gz_alg_update(gz_alg* a, uint32_t src, uint32_t dst, int) {
+  return guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    GZ_REQUIRE(src < a->a->n && dst < a->a->n, "vertex id out of range");
+    a->a->pre_insert(src, dst);
+    a->a->apply_update_batch(src, &dst, 1);
+    a->a->apply_update_batch(dst, &src, 1);
+  });
+}
+int gz_alg_ingest_pairs_device(gz_alg* a, const uint32_t* pairs, size_t count) {
+  return guarded([&] {
+    GZ_REQUIRE(a && (pairs || !count), "null argument");
+    a->a->ingest_device(pairs, count);
+  });
+}
+int gz_alg_ingest_pairs(gz_alg* a, const uint32_t* pairs, size_t count) {
+  return guarded([&] {
+    GZ_REQUIRE(a && (pairs || !count), "null argument");
+    const uint32_t n = a->a->n;
+    for (size_t i = 0; i < 2 * count; ++i)
+      if (pairs[i] >= n) throw GzError(GZ_ERR_INVALID_ARG, "vertex id out of range");
+    a->a->ingest_host(pairs, count);
+  });
+}
+int gz_alg_has_cached_query(gz_alg* a, int code, int* out) {
+  return guarded([&] {
+    GZ_REQUIRE(a && out, "null argument");
+    *out = a->a->has_cached_query(code) ? 1 : 0;
+  });
+}
+int gz_alg_prep_query(gz_alg* a, int code) {
+  return guarded([&] { GZ_REQUIRE(a, "null alg"); a->a->prep_query(code); });
+}
+int gz_alg_connected_components(gz_alg* a, uint32_t* parent_out, uint32_t* num_cc) {
+  return guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    uint32_t c = a->a->connected_components(parent_out);
+    if (num_cc) *num_cc = c;
+  });
+}
+int gz_alg_point_query(gz_alg* a, uint32_t u, uint32_t v, int* connected) {
+  return guarded([&] {
+    GZ_REQUIRE(a && connected, "null argument");
+    uint8_t r = 0;
+    a->a->point_queries(&u, &v, 1, &r);
+    *connected = r;
+  });
+}
+int gz_alg_point_queries(gz_alg* a, const uint32_t* us, const uint32_t* vs, size_t count, uint8_t* connected) {
+  return guarded([&] {
+    GZ_REQUIRE(a && ((us && vs && connected) || !count), "null argument");
+    a->a->point_queries(us, vs, count, connected);
+  });
+}
+int gz_alg_calc_spanning_forest(gz_alg* a, uint32_t* edges, size_t cap, size_t* num_edges) {
+  return guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    size_t k = a->a->spanning_forest(edges, cap);
+    if (num_edges) *num_edges = k;
+  });
+}
+int gz_alg_write_binary(gz_alg* a, const char* path) {
+  return guarded([&] {
+    GZ_REQUIRE(a && path, "null argument");
+    GzAlg& A = *a->a;
+    std::ofstream f(path, std::ios::binary | std::ios::trunc);
+    if (!f) throw GzError(GZ_ERR_IO, std::string("cannot open ") + path);
+    const uint64_t seed = A.seed;
+    const uint32_t nv = A.n;
+    const double sf = A.cfg.sketches_factor;
+    f.write((const char*)&seed, 8);
+    f.write((const char*)&nv, 4);
+    f.write((const char*)&sf, 8);
+    const size_t sb = A.sketch_bytes();
+    const uint32_t chunk = (uint32_t)std::max<size_t>(1, (size_t(64) << 20) / sb);
+    std::vector<uint8_t> buf((size_t)std::min(chunk, nv) * sb);
+    for (uint32_t v = 0; v < nv; v += chunk) {
+      const uint32_t c = std::min(chunk, nv - v);
+      A.dump(v, c, buf.data());
+      f.write((const char*)buf.data(), (std::streamsize)((size_t)c * sb));
+    }
+    if (!f) throw GzError(GZ_ERR_IO, std::string("write failed: ") + path);
+  });
+}
+int gz_alg_construct_from_serialized_data(const char* path, const gz_alg_config* cfg, gz_alg** out) {
+  return guarded([&] {
+    GZ_REQUIRE(path && out, "null argument");
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw GzError(GZ_ERR_IO, std::string("cannot open ") + path);
+    uint64_t seed = 0;
+    uint32_t nv = 0;
+    double sf = 1;
+    f.read((char*)&seed, 8);
+    f.read((char*)&nv, 4);
+    f.read((char*)&sf, 8);
+    if (!f) throw GzError(GZ_ERR_IO, "truncated header");
+    gz_alg_config c;
+    gz_alg_config_default(&c);
+    if (cfg) c = *cfg;
+    c.sketches_factor = sf;  // cc_sketch_alg.cpp:41
+    auto* A = new GzAlg(nv, seed, c);
+    try {
+      const size_t sb = A->sketch_bytes();
+      const uint32_t chunk = (uint32_t)std::max<size_t>(1, (size_t(64) << 20) / sb);
+      std::vector<uint8_t> buf((size_t)std::min(chunk, nv) * sb);
+      for (uint32_t v = 0; v < nv; v += chunk) {
+        const uint32_t cnt = std::min(chunk, nv - v);
+        f.read((char*)buf.data(), (std::streamsize)((size_t)cnt * sb));
+        if (!f) throw GzError(GZ_ERR_IO, "truncated sketch data");
+        A->load_packed(v, cnt, buf.data());
+      }
+      A->dsu_valid = false;  // cc_sketch_alg.cpp:62-63
+    } catch (...) {
+      delete A;
+      throw;
+    }
+    *out = new gz_alg{A};
+  });
+}
+int gz_alg_dump_sketches(gz_alg* a, uint32_t v0, uint32_t count, void* out, size_t bytes) {
+  return guarded([&] {
+    GZ_REQUIRE(a && out, "null argument");
+    GZ_REQUIRE(bytes >= (size_t)count * a->a->sketch_bytes(), "output buffer too small");
+    a->a->dump(v0, count, out);
+  });
+}
+size_t gz_alg_last_query_rounds(const gz_alg* a) { return a ? a->a->last_rounds : 0; }
+uint64_t gz_alg_get_total_updates(const gz_alg* a) { return a ? a->a->total_updates.load() : 0; }
+uint64_t gz_alg_footprint_bytes(uint32_t n, const gz_alg_config* cfg) {
+  gz_alg_config c;
+  gz_alg_config_default(&c);
+  if (cfg) c = *cfg;
+  if (n < 2) return 0;
+  return GzAlg::footprint(n, c);
+}
+
+}  // extern "C"

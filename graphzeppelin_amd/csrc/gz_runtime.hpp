@@ -1,0 +1,219 @@
+// gz_runtime.hpp -- host-side runtime objects behind the C ABI: device sketch slabs, staging,
+// the eager DSU and the Borůvka query loop.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gz_hip.h"
+#include "gz_kernels.hpp"
+
+namespace gz {
+
+struct GzError : std::runtime_error {
+  int code;
+  GzError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define GZ_HIP(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess)                                                                           \
+      throw ::gz::GzError(e_ == hipErrorOutOfMemory ? GZ_ERR_OOM : GZ_ERR_DEVICE,                   \
+                          std::string(#x) + " failed: " + hipGetErrorString(e_));                   \
+  } while (0)
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    GZ_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Owning pinned host buffer.
+template <class T>
+struct PinBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  PinBuf() = default;
+  PinBuf(const PinBuf&) = delete;
+  PinBuf& operator=(const PinBuf&) = delete;
+  PinBuf(PinBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  ~PinBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    GZ_HIP(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+int device_cu_count();
+void require_device();
+
+// Standalone Sketch (include/sketch.h:40-191) with its buckets in HBM.
+class GzSketch {
+ public:
+  Geo geo{};
+  int gb;
+  size_t sample_idx = 0;
+  hipStream_t st = nullptr;
+  DevBuf<uint64_t> alpha, col_bf, raw;
+  DevBuf<uint8_t> gamma;
+  DevBuf<uint32_t> dsts, out;
+  DevBuf<Tile> tiles;
+  DevBuf<uint8_t> packed;
+
+  GzSketch(uint64_t vector_len, uint64_t seed, size_t samples, size_t cols, int gamma_bytes);
+  ~GzSketch();
+  void update_batch(const uint64_t* idx, size_t count);
+  int sample(uint64_t* idx);
+  void merge(const GzSketch& o);
+  void range_merge(const GzSketch& o, size_t start, size_t n_samples);
+  void merge_raw(const void* raw_buckets);
+  void zero();
+  void serialize(void* out, size_t bytes) const;
+  void deserialize(const void* in, size_t bytes);
+  size_t bytes() const { return (size_t)geo.nb * (8 + gb); }
+};
+
+// CCSketchAlg (include/cc_sketch_alg.h) with every vertex sketch resident in HBM.
+class GzAlg {
+ public:
+  GzAlg(uint32_t n, uint64_t seed, const gz_alg_config& cfg);
+  ~GzAlg();
+
+  uint32_t n;
+  uint64_t seed;
+  gz_alg_config cfg;
+  Geo geo{};
+  int gb;
+  uint32_t tile;
+  uint32_t grid_cap;
+  hipStream_t st = nullptr;
+
+  // sketch slab
+  DevBuf<uint64_t> alpha, col_bf;
+  DevBuf<uint8_t> gamma;
+  // DSU / forest
+  DevBuf<uint32_t> parent, sf_list, flags, labels, qa, qb;
+  DevBuf<uint64_t> sf_hash;
+  DevBuf<uint8_t> qout;
+  DsuDev dsu{};
+  // Borůvka round buffers
+  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members, r_totals, r_done;
+  DevBuf<uint64_t> r_partials, r_acc_a, r_acc_g;
+  RoundBufs rb{};
+  // device stream ingest (pending pairs -> grouping -> tiles)
+  size_t cap_pairs;
+  size_t pending_n = 0;
+  DevBuf<uint32_t> pending, dst_arr, g_cnt, g_off, g_toff, g_cursor, g_totals, work_counter;
+  DevBuf<uint64_t> g_partials;
+  DevBuf<Tile> g_tiles;
+  DevBuf<uint8_t> merged;
+  GroupBufs gbufs{};
+  // host staging for apply_update_batch (batched form: two buffers, plain form: a ring)
+  struct Stage {
+    PinBuf<uint8_t> host;  // [tiles | dsts]
+    DevBuf<uint8_t> dev;
+    hipEvent_t ev = nullptr;
+    bool inflight = false;
+    std::vector<uint32_t> ent_v, ent_off, ent_cnt;
+    size_t used = 0;       // destinations staged
+    size_t dcap = 0;       // destination capacity
+    size_t max_tiles = 0;  // tile capacity
+  };
+  static constexpr size_t kMaxEntries = 65536;
+  std::mutex stage_mtx;
+  std::vector<Stage> stages;
+  size_t cur_stage = 0;
+  size_t stage_cap;
+  std::vector<uint32_t> vstamp, vtiles;
+  uint32_t epoch = 0;
+  // eager staging (pre_insert)
+  std::mutex eager_mtx;
+  PinBuf<uint32_t> eager_host;
+  DevBuf<uint32_t> eager_dev;
+  DevBuf<uint8_t> eager_merged;
+  hipEvent_t eager_ev = nullptr;
+  size_t eager_used = 0, eager_cap;
+  PinBuf<uint32_t> hflags;
+  PinBuf<uint32_t> hsmall;
+
+  std::atomic<bool> dsu_valid{true};
+  std::atomic<bool> update_locked{false};
+  size_t last_rounds = 0;
+  std::atomic<uint64_t> total_updates{0};
+  std::mutex query_mtx;
+
+  size_t sketch_bytes() const { return (size_t)geo.nb * (8 + gb); }
+  size_t desired_updates_per_batch() const {
+    size_t num = sketch_bytes() / sizeof(uint32_t);
+    return (size_t)(num * cfg.batch_factor);
+  }
+  static uint64_t footprint(uint32_t n, const gz_alg_config& cfg);
+
+  void pre_insert(uint32_t src, uint32_t dst);
+  void apply_update_batch(uint32_t src, const uint32_t* dsts, size_t k);
+  void apply_raw(uint32_t src, const void* raw);
+  void ingest_device(const uint32_t* pairs_dev, size_t count);
+  void ingest_host(const uint32_t* pairs, size_t count);
+  bool has_cached_query(int code);
+  void prep_query(int code);
+  uint32_t connected_components(uint32_t* parent_out);
+  void point_queries(const uint32_t* us, const uint32_t* vs, size_t count, uint8_t* out);
+  size_t spanning_forest(uint32_t* edges, size_t cap);
+  void dump(uint32_t v0, uint32_t count, void* out);
+  void load_packed(uint32_t v0, uint32_t count, const void* in);
+  void flush_all();
+
+ private:
+  void submit_stage_locked();
+  void plain_launch(uint32_t src, const uint32_t* dsts, size_t k);
+  void flush_pending();
+  void submit_eager_locked();
+  void sync_eager_state();
+  void ensure_answer();
+  void boruvka();
+  uint32_t read_flag(int slot);
+};
+
+}  // namespace gz

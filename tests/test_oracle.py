@@ -1,0 +1,280 @@
+"""Pins the CPU oracle (oracle/gz_oracle.cpp) before it is trusted as the checker.
+
+Each check is chosen so that a plausible restatement mistake fails it: golden XXH3 vectors from the
+xxhash library, the integer geometry table, the reference's known answers
+(test/sketch_test.cpp:14-18, 66-80, 423-458; test/cc_alg_test.cpp:41-57, 149-176, 223-263), the
+reference fixtures (78 components), linearity and exact connectivity over the final edge set.
+"""
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from graphzeppelin_amd import streams
+
+MASK64 = (1 << 64) - 1
+
+
+def test_xxh3_golden_vectors(oracle_mod, golden_dir):
+    with open(os.path.join(golden_dir, "xxh3_vectors.json")) as f:
+        vecs = json.load(f)["vectors"]
+    assert len(vecs) > 400
+    idx = np.array([int(v[0]) for v in vecs], dtype=np.uint64)
+    seed = np.array([int(v[1]) for v in vecs], dtype=np.uint64)
+    want = np.array([int(v[2]) for v in vecs], dtype=np.uint64)
+    got = oracle_mod.xxh3_len8_many(idx, seed)
+    assert np.array_equal(got, want)
+
+
+def test_xxh3_matches_library_random(oracle_mod):
+    xxhash = pytest.importorskip("xxhash")
+    rng = np.random.default_rng(7)
+    idx = rng.integers(0, 2**63, size=2000, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, 2000, dtype=np.uint64)
+    seed = rng.integers(0, 2**63, size=2000, dtype=np.uint64)
+    got = oracle_mod.xxh3_len8_many(idx, seed)
+    for i in range(0, 2000, 37):
+        assert int(got[i]) == xxhash.xxh3_64_intdigest(struct.pack("<Q", int(idx[i])), seed=int(seed[i]))
+
+
+# (n, bkt_per_col D, num_samples C, num_buckets) from include/sketch.h:62-76,181 with factor 1
+GEOMETRY = [(1024, 20, 18, 361), (1 << 12, 24, 18, 433), (1 << 15, 30, 21, 631), (1 << 17, 34, 24, 817),
+            (1 << 18, 36, 25, 901), (1 << 20, 40, 28, 1121)]
+
+
+@pytest.mark.parametrize("n,D,C,nb", GEOMETRY)
+def test_geometry_table(oracle_mod, n, D, C, nb):
+    vl = oracle_mod.calc_vector_length(n)
+    assert vl == n * (n - 1) // 2
+    assert oracle_mod.calc_bkt_per_col(vl) == D
+    assert oracle_mod.calc_cc_samples(n, 1.0) == C
+    alg_sk = oracle_mod.Sketch(vl, 42, C, 1)
+    assert alg_sk.get_buckets() == nb
+    assert alg_sk.bucket_array_bytes() == nb * 12
+
+
+def test_sample_results_zero_and_out_of_samples(oracle_mod):
+    # test/sketch_test.cpp:14-18
+    s = oracle_mod.Sketch(10, 12345, 1, 7)
+    assert s.sample()[1] == oracle_mod.ZERO
+    s.update(1)
+    with pytest.raises(oracle_mod.OracleOutOfSamples):
+        s.sample()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 42, 987654321])
+def test_index_zero_updated_three_times(oracle_mod, seed):
+    # test/sketch_test.cpp:66-80
+    s = oracle_mod.Sketch(40, seed, 1, 7)
+    for _ in range(3):
+        s.update(0)
+    assert s.sample() == (0, oracle_mod.GOOD)
+
+
+def _image(oracle_mod, vl, seed, samples, cols, idx, gb=4):
+    s = oracle_mod.Sketch(vl, seed, samples, cols, gb)
+    s.update_batch(idx)
+    return s.serialize()
+
+
+def test_linearity_and_cancellation(oracle_mod):
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 1 << 40, 3000, dtype=np.uint64)
+    b = rng.integers(0, 1 << 40, 3000, dtype=np.uint64)
+    b[:500] = a[:500]  # shared indices cancel in the symmetric difference
+    ia = np.frombuffer(_image(oracle_mod, 1 << 40, 9, 5, 2, a), dtype=np.uint8)
+    ib = np.frombuffer(_image(oracle_mod, 1 << 40, 9, 5, 2, b), dtype=np.uint8)
+    sym = np.setxor1d(a, b)
+    isym = np.frombuffer(_image(oracle_mod, 1 << 40, 9, 5, 2, sym), dtype=np.uint8)
+    assert np.array_equal(ia ^ ib, isym)
+    both = np.concatenate([a, a])
+    assert not np.any(np.frombuffer(_image(oracle_mod, 1 << 40, 9, 5, 2, both), dtype=np.uint8))
+
+
+@pytest.mark.parametrize("gb", [4, 8])
+def test_single_update_image_from_library_hashes(oracle_mod, gb):
+    """One update touches exactly: the deterministic bucket (last) and, in column c, depth
+    ctz(XXH3(idx, seed + 5c) | 1 << D) unless that equals D -- hashes from the xxhash library."""
+    xxhash = pytest.importorskip("xxhash")
+    vl, seed, samples = 1 << 20, 42, 6
+    idx = (5 << 32) | 77
+    img = np.frombuffer(_image(oracle_mod, vl, seed, samples, 1, [idx], gb), dtype=np.uint8)
+    D = oracle_mod.calc_bkt_per_col(vl)
+    nb = samples * D + 1
+    rec = 8 + gb
+    h = lambda s: xxhash.xxh3_64_intdigest(struct.pack("<Q", idx), seed=s)
+    chk = h(seed) & ((1 << (8 * gb)) - 1)
+    want = bytearray(nb * rec)
+    entry = struct.pack("<Q", idx) + chk.to_bytes(gb, "little")
+    want[(nb - 1) * rec:nb * rec] = entry
+    for c in range(samples):
+        d = ((h(seed + 5 * c) | (1 << D)) & -(h(seed + 5 * c) | (1 << D))).bit_length() - 1
+        if d < D:
+            want[(c * D + d) * rec:(c * D + d + 1) * rec] = entry
+    assert img.tobytes() == bytes(want)
+
+
+def test_depth_distribution_is_geometric(oracle_mod):
+    # one column, many distinct indices: bucket d must hold about 2^-(d+1) of them; this fails for a
+    # leading-zero count, a wrong shift, or an OR with the wrong bit
+    vl = 1 << 30
+    D = oracle_mod.calc_bkt_per_col(vl)
+    cnt = np.zeros(D + 1)
+    seed = 5
+    idx = np.arange(1, 20001, dtype=np.uint64) * np.uint64(2654435761)
+    h = oracle_mod.xxh3_len8_many(idx, np.uint64(seed))
+    for x in h.tolist():
+        d = min(((x | (1 << D)) & -(x | (1 << D))).bit_length() - 1, D)
+        cnt[d] += 1
+    frac = cnt / cnt.sum()
+    for d in range(5):
+        assert abs(frac[d] - 2.0 ** -(d + 1)) < 0.02
+
+
+def test_raw_bucket_merge_twice_is_zero(oracle_mod):
+    # test/sketch_test.cpp:423-458
+    for t in range(5):
+        s1 = oracle_mod.Sketch(4096, 100 + 5 * t, 1, 1)
+        s2 = oracle_mod.Sketch(4096, 100 + 5 * t, 1, 1)
+        s1.update_batch(np.arange(1024, dtype=np.uint64))
+        raw = s1.serialize()
+        s2.merge_raw_bucket_buffer(raw)
+        idx, res = s2.sample()
+        assert res != oracle_mod.ZERO
+        if res == oracle_mod.GOOD:
+            assert 0 <= idx < 1024
+        s2.merge_raw_bucket_buffer(raw)
+        s2.reset_sample_state()
+        assert s2.sample()[1] == oracle_mod.ZERO
+
+
+def test_sample_grinder_invariants(oracle_mod):
+    # test/sketch_test.cpp:364-421 (insert and delete grinders) with a fixed seed
+    nodes = 1024
+    vl = oracle_mod.calc_vector_length(nodes)
+    samples = oracle_mod.calc_cc_samples(nodes, 1)
+    s = oracle_mod.Sketch(vl, 31337, samples)
+    ins = [(src << 32) | dst for src in range(nodes - 1) for dst in range(src + 7, nodes, 7)]
+    dele = [(src << 32) | dst for src in range(1, nodes - 1, 2) for dst in range(src + 7, nodes, 7)]
+    s.update_batch(ins)
+    s.update_batch(dele)
+    ok = 0
+    for _ in range(samples):
+        idx, res = s.sample()
+        if res == oracle_mod.FAIL:
+            continue
+        assert res == oracle_mod.GOOD
+        src, dst = idx >> 32, idx & 0xFFFFFFFF
+        assert (dst - src) % 7 == 0 and src % 2 == 0
+        ok += 1
+    assert ok >= 2
+
+
+def test_fixture_graph_has_78_components(oracle_mod, golden_dir):
+    # test/cc_alg_test.cpp:41-57 and test/util/graph_verifier_test.cpp:27-31
+    n, edges = streams.read_ascii_graph(os.path.join(golden_dir, "multiples_graph_1024.txt"))
+    assert n == 1024 and edges.shape[0] == 5206
+    c, lab = oracle_mod.exact_cc(n, edges)
+    assert c == 78
+    alg = oracle_mod.CCAlg(n, 777)
+    alg.ingest_pairs(edges)
+    par, ncc = alg.connected_components()
+    assert ncc == 78
+    assert np.array_equal(oracle_mod.canonical(par), lab)
+
+
+def test_fixture_stream_final_state(oracle_mod, golden_dir):
+    # test/res/multiples_graph_1024_stream.data: final state equals the ASCII graph
+    n, pairs, types = streams.read_binary_stream(os.path.join(golden_dir, "multiples_graph_1024_stream.data"))
+    assert n == 1024 and pairs.shape[0] == 8528
+    assert int((types == 0).sum()) == 6867 and int((types == 1).sum()) == 1661
+    streams.check_type_consistent(pairs, types)
+    _, edges = streams.read_ascii_graph(os.path.join(golden_dir, "multiples_graph_1024.txt"))
+    assert np.array_equal(np.sort(streams.edge_ids(streams.final_edge_set(pairs))), np.sort(streams.edge_ids(edges)))
+    alg = oracle_mod.CCAlg(n, 4242)
+    alg.ingest_pairs(pairs)
+    par, ncc = alg.connected_components()
+    assert ncc == 78
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_dynamic_streams_match_exact_dsu(oracle_mod, seed):
+    # test/cc_alg_test.cpp:59-95 with seeded streams instead of clock seeds
+    for density in (0.03, 0.002):
+        pairs, types = streams.dynamic_erdos_stream(1024, density, 0.5, 0.05, 3, seed)
+        streams.check_type_consistent(pairs, types)
+        alg = oracle_mod.CCAlg(1024, 1000 + seed)
+        alg.ingest_pairs(pairs)
+        par, ncc = alg.connected_components()
+        c, lab = oracle_mod.exact_cc(1024, pairs)
+        assert ncc == c
+        assert np.array_equal(oracle_mod.canonical(par), lab)
+
+
+def test_point_query_grid(oracle_mod):
+    # test/cc_alg_test.cpp:149-176 on the fixture-like structure: 10x10 grid with one cut column
+    e = streams.grid_graph_edges(10, 10)
+    e = e[~((e[:, 0] % 10 == 4) & (e[:, 1] % 10 == 5))]  # cut between columns 4 and 5
+    alg = oracle_mod.CCAlg(100, 99)
+    alg.ingest_pairs(e)
+    par, ncc = alg.connected_components()
+    assert ncc == 2
+    lab = oracle_mod.canonical(par)
+    for i in range(0, 100, 7):
+        for j in range(0, 100, 11):
+            assert alg.point_query(i, j) == (lab[i] == lab[j])
+
+
+def test_eager_dsu_state_sequence(oracle_mod):
+    # test/cc_alg_test.cpp:223-263
+    alg = oracle_mod.CCAlg(100, 5)
+    steps = [((1, 2), True), ((2, 3), True), ((1, 3), True), ((1, 3), True), ((2, 3), False), ((2, 3), True)]
+    for (a, b), cached in steps:
+        alg.update(a, b)
+        assert alg.has_cached_query() == cached
+        par, ncc = alg.connected_components()
+        assert alg.has_cached_query()
+    lab = oracle_mod.canonical(par)
+    assert lab[1] == lab[2] == lab[3] and ncc == 98
+
+
+def test_batching_independence(oracle_mod):
+    # the byte image does not depend on how updates are grouped into apply_update_batch calls
+    pairs, _ = streams.erdos_renyi_churn_stream(256, 0.1, 5)
+    a = oracle_mod.CCAlg(256, 42)
+    a.ingest_pairs(pairs)
+    b = oracle_mod.CCAlg(256, 42)
+    b.allocate_worker_memory(1)
+    rng = np.random.default_rng(0)
+    by_src = {}
+    for s, d in pairs.tolist():
+        by_src.setdefault(s, []).append(d)
+        by_src.setdefault(d, []).append(s)
+    for v, ds in by_src.items():
+        ds = np.array(ds, dtype=np.uint32)
+        cuts = np.sort(rng.integers(0, len(ds) + 1, 3))
+        for part in np.split(ds, cuts):
+            if part.size:
+                b.apply_update_batch(0, v, part)
+    assert np.array_equal(a.dump(), b.dump())
+
+
+def test_config1_stream_shape():
+    pairs, types = streams.erdos_renyi_churn_stream(1024, 0.1, 1)
+    streams.check_type_consistent(pairs, types)
+    m = int((types == 0).sum() - (types == 1).sum())
+    n_ins_first = pairs.shape[0] - int((types == 1).sum()) * 1.1
+    assert 51000 < m + int((types == 1).sum()) * 0.9 < 54000 or n_ins_first > 0
+    assert 60000 < pairs.shape[0] < 68000
+
+
+def test_cpu_baseline_trace_matches_exact(oracle_mod):
+    pairs, _ = streams.erdos_renyi_churn_stream(512, 0.05, 11)
+    alg = oracle_mod.CCAlg(512, 3)
+    qpos = np.array([pairs.shape[0] // 3, pairs.shape[0]], dtype=np.uint64)
+    wall, lat, ans = alg.baseline_trace(pairs, qpos, [1, 1], [0, 0], [0, 0], 4)
+    c1, _ = oracle_mod.exact_cc(512, pairs[: pairs.shape[0] // 3])
+    c2, _ = oracle_mod.exact_cc(512, pairs)
+    assert list(ans) == [c1, c2]
+    assert wall > 0 and np.all(lat > 0)
