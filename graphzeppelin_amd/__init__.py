@@ -14,6 +14,12 @@ from ._lib import (  # noqa: F401
     CCAlgConfiguration,
     CCSketchAlg,
     ConnectedComponents,
+    REQUEST_DTYPE,
+    REQ_CC,
+    REQ_POINT,
+    REQ_UPDATES,
+    Server,
+    digest_of_image,
     GZError,
     OutOfSamplesException,
     Sketch,

@@ -53,6 +53,49 @@ class _AlgConfig(ctypes.Structure):
     ]
 
 
+class Request(ctypes.Structure):
+    """gz_request (include/gz_serving.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_uint32),
+        ("tenant", ctypes.c_uint32),
+        ("offset", ctypes.c_uint64),
+        ("count", ctypes.c_uint64),
+        ("u", ctypes.c_uint32),
+        ("v", ctypes.c_uint32),
+        ("arrival_s", ctypes.c_double),
+    ]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double), ("alg_bytes", ctypes.c_double),
+                ("items", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AlgProfile(ctypes.Structure):
+    _fields_ = [("update", KernelStat), ("boruvka", KernelStat)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("wall_s", ctypes.c_double),
+        ("updates", ctypes.c_uint64),
+        ("queries", ctypes.c_uint64),
+        ("boruvka_runs", ctypes.c_uint64),
+        ("cached_answers", ctypes.c_uint64),
+        ("query_groups", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_UPDATES, REQ_POINT, REQ_CC = 0, 1, 2
+REQUEST_DTYPE = np.dtype([("kind", "<u4"), ("tenant", "<u4"), ("offset", "<u8"), ("count", "<u8"), ("u", "<u4"),
+                          ("v", "<u4"), ("arrival_s", "<f8")], align=True)
+
 _lib = None
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -113,6 +156,22 @@ _SIGS = {
     "gz_alg_last_query_rounds": (_sz, [_vp]),
     "gz_alg_get_total_updates": (ctypes.c_uint64, [_vp]),
     "gz_alg_footprint_bytes": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.POINTER(_AlgConfig)]),
+    "gz_alg_digest": (ctypes.c_int, [_vp, _u64p]),
+    "gz_alg_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_alg_get_profile": (ctypes.c_int, [_vp, ctypes.POINTER(AlgProfile)]),
+    # include/gz_serving.h
+    "gz_alg_serve_trace": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _sz, _vp, _vp, ctypes.POINTER(ServeStats)]),
+    "gz_alg_plain_trace": (ctypes.c_int, [_vp, _vp, _vp, _sz, _vp, _vp, ctypes.POINTER(ServeStats)]),
+    "gz_server_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "gz_server_destroy": (ctypes.c_int, [_vp]),
+    "gz_server_admit": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_AlgConfig),
+                                       ctypes.POINTER(ctypes.c_int)]),
+    "gz_server_release": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_server_tenant_alg": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "gz_server_reserved_bytes": (ctypes.c_uint64, [_vp]),
+    "gz_server_budget_bytes": (ctypes.c_uint64, [_vp]),
+    "gz_server_run_trace": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _sz, ctypes.POINTER(_AlgConfig), ctypes.c_int,
+                                           _vp, _vp, _vp, ctypes.POINTER(ServeStats)]),
 }
 
 
@@ -126,6 +185,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m graphzeppelin_amd.build`")
+        # torch bundles its own HIP runtime with the same soname (libamdhip64.so.7).  Loading torch
+        # first lets the library bind to that copy, so one process never holds two HIP runtimes
+        # (a second runtime cannot open the device).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
@@ -458,3 +524,110 @@ class CCSketchAlg:
 
     def get_total_updates(self) -> int:
         return int(lib().gz_alg_get_total_updates(self._h))
+
+    def set_profiling(self, enable: bool = True):
+        _check(lib().gz_alg_set_profiling(self._h, 1 if enable else 0))
+
+    def get_profile(self) -> dict:
+        p = AlgProfile()
+        _check(lib().gz_alg_get_profile(self._h, ctypes.byref(p)))
+        return {"update": p.update.as_dict(), "boruvka": p.boruvka.as_dict()}
+
+    def digest(self) -> int:
+        d = ctypes.c_uint64()
+        _check(lib().gz_alg_digest(self._h, ctypes.byref(d)))
+        return int(d.value)
+
+    def serve_trace(self, pairs_ptr: int, on_device: bool, reqs: np.ndarray):
+        """Batched/served replay (gz_alg_serve_trace).  `reqs` has REQUEST_DTYPE; returns
+        (latency seconds, answers, stats dict)."""
+        r = np.ascontiguousarray(reqs, dtype=REQUEST_DTYPE)
+        lat = np.zeros(r.size, dtype=np.float64)
+        ans = np.zeros(r.size, dtype=np.uint32)
+        st = ServeStats()
+        _check(lib().gz_alg_serve_trace(self._h, ctypes.c_void_p(pairs_ptr), 1 if on_device else 0, _ptr(r), r.size,
+                                        _ptr(lat), _ptr(ans), ctypes.byref(st)))
+        return lat, ans, st.as_dict()
+
+    def plain_trace(self, pairs: np.ndarray, reqs: np.ndarray):
+        """Reference-shaped replay (gz_alg_plain_trace) over host pairs."""
+        p = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint32).reshape(-1, 2))
+        r = np.ascontiguousarray(reqs, dtype=REQUEST_DTYPE)
+        lat = np.zeros(r.size, dtype=np.float64)
+        ans = np.zeros(r.size, dtype=np.uint32)
+        st = ServeStats()
+        _check(lib().gz_alg_plain_trace(self._h, _ptr(p), _ptr(r), r.size, _ptr(lat), _ptr(ans), ctypes.byref(st)))
+        return lat, ans, st.as_dict()
+
+
+def digest_of_image(image: np.ndarray, num_buckets: int, gamma_bytes: int = 4) -> int:
+    """The gz_alg_digest formula over a packed byte image (numpy, mod 2^64)."""
+    rec = 8 + gamma_bytes
+    raw = np.ascontiguousarray(image, dtype=np.uint8).reshape(-1, rec)
+    alpha = raw[:, :8].copy().view("<u8").ravel()
+    gamma = raw[:, 8:].copy().view("<u4" if gamma_bytes == 4 else "<u8").ravel().astype(np.uint64)
+    pos = np.arange(alpha.size, dtype=np.uint64)
+
+    def fmix(k):
+        k = k ^ (k >> np.uint64(33))
+        k = k * np.uint64(0xFF51AFD7ED558CCD)
+        k = k ^ (k >> np.uint64(33))
+        k = k * np.uint64(0xC4CEB9FE1A85EC53)
+        return k ^ (k >> np.uint64(33))
+
+    with np.errstate(over="ignore"):
+        x = alpha ^ fmix(gamma ^ (pos * np.uint64(0x9E3779B97F4A7C15)))
+        return int(np.sum(fmix(x + pos), dtype=np.uint64))
+
+
+class Server:
+    """Multi-tenant request loop (include/gz_serving.h)."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, workers: int = 4):
+        h = ctypes.c_void_p()
+        _check(lib().gz_server_create(hbm_budget_bytes, workers, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gz_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def budget_bytes(self) -> int:
+        return int(lib().gz_server_budget_bytes(self._h))
+
+    @property
+    def reserved_bytes(self) -> int:
+        return int(lib().gz_server_reserved_bytes(self._h))
+
+    def admit(self, n: int, seed: int, config: CCAlgConfiguration | None = None) -> int:
+        t = ctypes.c_int(-1)
+        c = (config or CCAlgConfiguration()).to_c()
+        _check(lib().gz_server_admit(self._h, n, seed, ctypes.byref(c), ctypes.byref(t)))
+        return t.value
+
+    def release(self, tenant: int):
+        _check(lib().gz_server_release(self._h, tenant))
+
+    def run_trace(self, reqs: np.ndarray, tenant_ptrs, tenant_n, tenant_seed, config=None, release_when_done=True):
+        r = np.ascontiguousarray(reqs, dtype=REQUEST_DTYPE)
+        T = len(tenant_n)
+        ptrs = (ctypes.c_void_p * T)(*[ctypes.c_void_p(int(p)) for p in tenant_ptrs])
+        tn = np.ascontiguousarray(tenant_n, dtype=np.uint32)
+        ts = np.ascontiguousarray(tenant_seed, dtype=np.uint64)
+        lat = np.zeros(r.size, dtype=np.float64)
+        ans = np.zeros(r.size, dtype=np.uint32)
+        dig = np.zeros(T, dtype=np.uint64)
+        st = ServeStats()
+        c = (config or CCAlgConfiguration()).to_c()
+        _check(lib().gz_server_run_trace(self._h, _ptr(r), r.size, ctypes.cast(ptrs, ctypes.c_void_p), _ptr(tn),
+                                         _ptr(ts), T, ctypes.byref(c), 1 if release_when_done else 0, _ptr(lat),
+                                         _ptr(ans), _ptr(dig), ctypes.byref(st)))
+        return lat, ans, dig, st.as_dict()

@@ -752,6 +752,47 @@ __global__ void k_packed(uint64_t* __restrict__ alpha, G* __restrict__ gamma, Ge
   }
 }
 
+// Order-sensitive digest of every vertex's bucket array:
+//   D = sum over vertices v, buckets b of fmix64(x + pos), pos = v*nb + b,
+//   x = alpha ^ fmix64(gamma ^ pos * 0x9E3779B97F4A7C15)   (all mod 2^64)
+// A sum is independent of reduction order, so the value is deterministic.
+template <class G>
+__global__ void __launch_bounds__(256)
+k_digest(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, unsigned long long* __restrict__ out) {
+  const size_t total = (size_t)g.n * g.nb;
+  uint64_t acc = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)(i / g.nb), b = (uint32_t)(i % g.nb);
+    const size_t o = (size_t)v * g.stride + b;
+    const uint64_t pos = i;
+    const uint64_t x = alpha[o] ^ mix64((uint64_t)gamma[o] ^ (pos * 0x9E3779B97F4A7C15ULL));
+    acc += mix64(x + pos);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = __shfl_xor((unsigned)(uint32_t)acc, o), hi = __shfl_xor((unsigned)(uint32_t)(acc >> 32), o);
+    acc += ((uint64_t)hi << 32) | lo;
+  }
+  if (lane_id() == 0) atomicAdd(out, (unsigned long long)acc);
+}
+
+// Algorithmic bytes of a grouped flush: sum over vertices of 4k + 2*min(S, k*(C+1)*B).
+__global__ void k_flush_alg_bytes(const uint32_t* __restrict__ cnt, uint32_t n, uint64_t S, uint64_t per_update,
+                                  unsigned long long* __restrict__ out) {
+  uint64_t acc = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint64_t k = cnt[v];
+    if (k) {
+      const uint64_t touched = k * per_update;
+      acc += 4 * k + 2 * (touched < S ? touched : S);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = __shfl_xor((unsigned)(uint32_t)acc, o), hi = __shfl_xor((unsigned)(uint32_t)(acc >> 32), o);
+    acc += ((uint64_t)hi << 32) | lo;
+  }
+  if (lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
 }  // namespace
 
 // ================================================================================================
@@ -856,6 +897,22 @@ void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void*
   else
     hipLaunchKernelGGL(k_reduce_sample<uint64_t>, dim3(grid), dim3(256), 0, s, rb.members, rb.root, rb.off, rb.cnt,
                        round, alpha, (const uint64_t*)gamma, g, d, rb.acc_a, rb.acc_g, rb.done);
+}
+
+void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s) {
+  launch_fill_u64(out, 0, 1, s);
+  const uint32_t grid = grid_for((size_t)g.n * g.nb, 256, 4096);
+  auto* o = reinterpret_cast<unsigned long long*>(out);
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_digest<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, o);
+  else
+    hipLaunchKernelGGL(k_digest<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, o);
+}
+
+void launch_flush_alg_bytes(const uint32_t* cnt, uint32_t n, uint64_t S, uint64_t per_update, uint64_t* out,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_flush_alg_bytes, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, cnt, n, S, per_update,
+                     reinterpret_cast<unsigned long long*>(out));
 }
 
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s) {

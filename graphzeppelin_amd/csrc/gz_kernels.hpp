@@ -82,6 +82,13 @@ void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t
 void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,
                           hipStream_t s);
 
+// order-sensitive 64-bit digest of all n sketches (see k_digest)
+void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s);
+
+// adds sum_v 4k + 2*min(S, k*per_update) over per-vertex counts k = cnt[v] to *out
+void launch_flush_alg_bytes(const uint32_t* cnt, uint32_t n, uint64_t S, uint64_t per_update, uint64_t* out,
+                            hipStream_t s);
+
 // ---- standalone sketch ops ----
 // out[0] = code, out[1..2] = idx (lo, hi)
 void launch_sketch_sample(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t sample_idx,

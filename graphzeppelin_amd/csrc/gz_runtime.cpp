@@ -254,6 +254,11 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
 
 GzAlg::~GzAlg() {
   if (st) (void)hipStreamSynchronize(st);
+  for (auto& t : timed) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : ev_pool) (void)hipEventDestroy(e);
   for (auto& s : stages)
     if (s.ev) (void)hipEventDestroy(s.ev);
   if (eager_ev) (void)hipEventDestroy(eager_ev);
@@ -357,8 +362,10 @@ void GzAlg::submit_stage_locked() {
   const size_t doff = s.max_tiles * sizeof(Tile);
   GZ_HIP(hipMemcpyAsync(s.dev.p, s.host.p, nt * sizeof(Tile), hipMemcpyHostToDevice, st));
   GZ_HIP(hipMemcpyAsync(s.dev.p + doff, s.host.p + doff, s.used * 4, hipMemcpyHostToDevice, st));
+  if (prof) prof_begin(0, batch_alg_bytes(s.ent_cnt), false, s.used);
   launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
                       reinterpret_cast<uint32_t*>(s.dev.p + doff), nullptr, alpha.p, gamma.p, geo, col_bf.p, st);
+  if (prof) prof_end();
   GZ_HIP(hipGetLastError());
   if (!s.ev) GZ_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
   GZ_HIP(hipEventRecord(s.ev, st));
@@ -368,6 +375,13 @@ void GzAlg::submit_stage_locked() {
   s.ent_off.clear();
   s.ent_cnt.clear();
   cur_stage = (cur_stage + 1) % stages.size();
+}
+
+double GzAlg::batch_alg_bytes(const std::vector<uint32_t>& counts) const {
+  const double S = (double)sketch_bytes(), per = (double)(geo.C + 1) * (8 + gb);
+  double b = 0;
+  for (uint32_t k : counts) b += 4.0 * k + 2.0 * std::min(S, k * per);
+  return b;
 }
 
 // Plain form: one launch per apply_update_batch call (one gutter flush).
@@ -390,8 +404,10 @@ void GzAlg::plain_launch(uint32_t src, const uint32_t* dsts, size_t k) {
     ht[t] = Tile{src, (uint32_t)(t * tile), (uint32_t)std::min<size_t>(tile, k - t * tile), nt > 1 ? 1u : 0u};
   std::memcpy(s.host.p + tbytes, dsts, k * 4);
   GZ_HIP(hipMemcpyAsync(s.dev.p, s.host.p, need, hipMemcpyHostToDevice, st));
+  if (prof) prof_begin(0, batch_alg_bytes(std::vector<uint32_t>{(uint32_t)k}), false, k);
   launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
                       reinterpret_cast<uint32_t*>(s.dev.p + tbytes), nullptr, alpha.p, gamma.p, geo, col_bf.p, st);
+  if (prof) prof_end();
   GZ_HIP(hipGetLastError());
   if (!s.ev) GZ_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
   GZ_HIP(hipEventRecord(s.ev, st));
@@ -410,11 +426,57 @@ void GzAlg::apply_raw(uint32_t src, const void* rawb) {
   GZ_HIP(hipStreamSynchronize(st));
 }
 
+// ---- kernel timing ----
+hipEvent_t GzAlg::take_event() {
+  if (!ev_pool.empty()) {
+    hipEvent_t e = ev_pool.back();
+    ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  GZ_HIP(hipEventCreate(&e));
+  return e;
+}
+hipEvent_t GzAlg::prof_begin(int kind, double host_bytes, bool dev_bytes, uint64_t items) {
+  Timed t{take_event(), take_event(), kind, host_bytes, dev_bytes, items};
+  GZ_HIP(hipEventRecord(t.a, st));
+  timed.push_back(t);
+  return t.a;
+}
+void GzAlg::prof_end() { GZ_HIP(hipEventRecord(timed.back().b, st)); }
+// Called after the stream has been synchronised: folds finished event pairs into the profile.
+void GzAlg::prof_collect() {
+  if (timed.empty()) return;
+  for (auto& t : timed) {
+    float ms = 0;
+    GZ_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+    gz_kernel_stat& k = t.kind == 0 ? profile.update : profile.boruvka;
+    k.launches += 1;
+    k.total_ms += ms;
+    k.alg_bytes += t.host_bytes;
+    k.items += t.items;
+    ev_pool.push_back(t.a);
+    ev_pool.push_back(t.b);
+  }
+  timed.clear();
+  uint64_t db = 0;
+  GZ_HIP(hipMemcpy(&db, prof_bytes.p, 8, hipMemcpyDeviceToHost));
+  profile.update.alg_bytes += (double)db;
+  launch_fill_u64(prof_bytes.p, 0, 1, st);
+  GZ_HIP(hipStreamSynchronize(st));
+}
+
 void GzAlg::flush_pending() {
   if (!pending_n) return;
   launch_group(pending.p, pending_n, n, tile, gbufs, st);
+  if (prof) {
+    const uint64_t per_update = (uint64_t)(geo.C + 1) * (8 + gb);
+    launch_flush_alg_bytes(g_cnt.p, n, sketch_bytes(), per_update, prof_bytes.p, st);
+    prof_begin(0, 0, true, 2 * pending_n);
+  }
   launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, work_counter.p, dst_arr.p, nullptr, alpha.p, gamma.p,
                       geo, col_bf.p, st);
+  if (prof) prof_end();
   GZ_HIP(hipGetLastError());
   total_updates += 2 * pending_n;
   pending_n = 0;
@@ -462,6 +524,8 @@ void GzAlg::flush_all() {
     if (cfg.batched) submit_stage_locked();
     flush_pending();
   }
+  GZ_HIP(hipStreamSynchronize(st));
+  if (prof) prof_collect();
   if (read_flag(kFlagBadInput)) {
     launch_fill_u32(flags.p + kFlagBadInput, 0, 1, st);
     GZ_HIP(hipStreamSynchronize(st));
@@ -508,8 +572,11 @@ void GzAlg::ensure_answer() {
 void GzAlg::boruvka() {
   update_locked = true;
   try {
+    const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
     launch_dsu_reset(dsu, st);
+    if (prof) prof_begin(1, round_bytes, false, 1);
     launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, st);
+    if (prof) prof_end();
     GZ_HIP(hipGetLastError());
     uint32_t round = 0;
     bool modified = read_flag(kFlagModified) != 0;
@@ -517,12 +584,15 @@ void GzAlg::boruvka() {
       ++round;
       if (round >= geo.samples)
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
+      if (prof) prof_begin(1, round_bytes, false, 1);
       launch_round(gb, round, alpha.p, gamma.p, geo, dsu, rb, st);
+      if (prof) prof_end();
       GZ_HIP(hipGetLastError());
       modified = read_flag(kFlagModified) != 0;
     }
     last_rounds = round;
     dsu_valid = true;
+    if (prof) prof_collect();
   } catch (...) {
     update_locked = false;
     dsu_valid = false;
@@ -588,6 +658,17 @@ void GzAlg::dump(uint32_t v0, uint32_t count, void* outp) {
   }
 }
 
+uint64_t GzAlg::digest() {
+  flush_all();
+  DevBuf<uint64_t> d;
+  d.alloc(1);
+  launch_digest(gb, alpha.p, gamma.p, geo, d.p, st);
+  uint64_t h = 0;
+  GZ_HIP(hipMemcpyAsync(&h, d.p, 8, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+  return h;
+}
+
 void GzAlg::load_packed(uint32_t v0, uint32_t count, const void* in) {
   const size_t sb = sketch_bytes();
   const uint32_t chunk = (uint32_t)std::max<size_t>(1, (size_t(64) << 20) / sb);
@@ -610,17 +691,9 @@ using gz::GzAlg;
 using gz::GzError;
 using gz::GzSketch;
 
-struct gz_sketch {
-  GzSketch* s;
-};
-struct gz_alg {
-  GzAlg* a;
-};
-
 static thread_local std::string g_last_error;
 
-template <class F>
-static int guarded(F&& f) {
+int gz::guarded_call(const std::function<void()>& f) {
   try {
     f();
     g_last_error.clear();
@@ -635,6 +708,11 @@ static int guarded(F&& f) {
     g_last_error = e.what();
     return GZ_ERR_DEVICE;
   }
+}
+
+template <class F>
+static int guarded(F&& f) {
+  return gz::guarded_call(std::function<void()>(std::forward<F>(f)));
 }
 
 #define GZ_REQUIRE(cond, msg)                                  \
@@ -909,6 +987,35 @@ int gz_alg_dump_sketches(gz_alg* a, uint32_t v0, uint32_t count, void* out, size
     GZ_REQUIRE(a && out, "null argument");
     GZ_REQUIRE(bytes >= (size_t)count * a->a->sketch_bytes(), "output buffer too small");
     a->a->dump(v0, count, out);
+  });
+}
+int gz_alg_set_profiling(gz_alg* a, int enable) {
+  return guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    GzAlg& A = *a->a;
+    A.flush_all();
+    if (!A.prof_bytes.p) {
+      A.prof_bytes.alloc(1);
+      gz::launch_fill_u64(A.prof_bytes.p, 0, 1, A.st);
+      GZ_HIP(hipStreamSynchronize(A.st));
+    }
+    A.prof = enable != 0;
+    A.profile = gz_alg_profile{};
+  });
+}
+int gz_alg_get_profile(gz_alg* a, gz_alg_profile* out) {
+  return guarded([&] {
+    GZ_REQUIRE(a && out, "null argument");
+    GzAlg& A = *a->a;
+    GZ_HIP(hipStreamSynchronize(A.st));
+    if (A.prof) A.prof_collect();
+    *out = A.profile;
+  });
+}
+int gz_alg_digest(gz_alg* a, uint64_t* out) {
+  return guarded([&] {
+    GZ_REQUIRE(a && out, "null argument");
+    *out = a->a->digest();
   });
 }
 size_t gz_alg_last_query_rounds(const gz_alg* a) { return a ? a->a->last_rounds : 0; }

@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -178,6 +179,24 @@ class GzAlg {
   PinBuf<uint32_t> hflags;
   PinBuf<uint32_t> hsmall;
 
+  // kernel timing (gz_alg_set_profiling)
+  struct Timed {
+    hipEvent_t a, b;
+    int kind;          // 0 update kernel, 1 Borůvka round
+    double host_bytes;  // algorithmic bytes known on the host (staged paths / rounds)
+    bool dev_bytes;     // bytes accumulated on the device (grouped flush)
+    uint64_t items;
+  };
+  bool prof = false;
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> ev_pool;
+  DevBuf<uint64_t> prof_bytes;  // device accumulator for grouped-flush bytes
+  gz_alg_profile profile{};
+  hipEvent_t prof_begin(int kind, double host_bytes, bool dev_bytes, uint64_t items);
+  void prof_end();
+  void prof_collect();
+  hipEvent_t take_event();
+
   std::atomic<bool> dsu_valid{true};
   std::atomic<bool> update_locked{false};
   size_t last_rounds = 0;
@@ -190,6 +209,7 @@ class GzAlg {
     return (size_t)(num * cfg.batch_factor);
   }
   static uint64_t footprint(uint32_t n, const gz_alg_config& cfg);
+  double batch_alg_bytes(const std::vector<uint32_t>& counts) const;
 
   void pre_insert(uint32_t src, uint32_t dst);
   void apply_update_batch(uint32_t src, const uint32_t* dsts, size_t k);
@@ -203,6 +223,7 @@ class GzAlg {
   size_t spanning_forest(uint32_t* edges, size_t cap);
   void dump(uint32_t v0, uint32_t count, void* out);
   void load_packed(uint32_t v0, uint32_t count, const void* in);
+  uint64_t digest();
   void flush_all();
 
  private:
@@ -216,4 +237,15 @@ class GzAlg {
   uint32_t read_flag(int slot);
 };
 
+// Runs f, converting exceptions to a gz_status and the thread's last-error message.
+int guarded_call(const std::function<void()>& f);
+
 }  // namespace gz
+
+// Opaque handle types of the C ABI.
+struct gz_sketch {
+  gz::GzSketch* s;
+};
+struct gz_alg {
+  gz::GzAlg* a;
+};

@@ -1,0 +1,181 @@
+"""Seeded generators for the large stream configurations, run with torch on the device.
+
+Streams of 10^8-10^9 updates with type-consistent deletes need a global dedup, which is a sort on
+the GPU and far too slow on the host.  torch is only the array engine here (synthetic input
+plumbing); nothing in the sketch path uses it.
+
+Construction of a typed stream (no self loops, an edge is inserted at most once, deleted at most
+once and only after its insert):
+  1. draw candidate edges from the model (R-MAT or Chung-Lu) in a seeded order;
+  2. drop self loops and keep the first occurrence of every edge, in draw order -> the inserts;
+  3. pick n_del distinct inserts to delete, each scheduled at a uniform position between its
+     insert and the end of the stream;
+  4. order everything by schedule key.
+Outputs are int32 (src, dst) pairs of shape (N, 2) and uint8 types (0 insert, 1 delete), on the
+device the generator ran on, plus the final edge list when asked for.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+_CHUNK = 1 << 25
+
+
+def _gen(seed: int, device) -> torch.Generator:
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) & 0x7FFFFFFFFFFFFFFF)
+    return g
+
+
+def rmat_candidates(scale: int, count: int, a: float, b: float, c: float, g: torch.Generator, device) -> torch.Tensor:
+    """Edge keys (min << 32 | max, int64) of `count` R-MAT draws; self loops are encoded as -1."""
+    out = torch.empty(count, dtype=torch.int64, device=device)
+    for s in range(0, count, _CHUNK):
+        m = min(_CHUNK, count - s)
+        src = torch.zeros(m, dtype=torch.int64, device=device)
+        dst = torch.zeros(m, dtype=torch.int64, device=device)
+        for _ in range(scale):
+            r = torch.rand(m, generator=g, device=device)
+            bit_s = (r >= a + b).to(torch.int64)
+            bit_d = ((r >= a) & (r < a + b) | (r >= a + b + c)).to(torch.int64)
+            src = src * 2 + bit_s
+            dst = dst * 2 + bit_d
+        lo, hi = torch.minimum(src, dst), torch.maximum(src, dst)
+        key = (lo << 32) | hi
+        key[src == dst] = -1
+        out[s:s + m] = key
+    return out
+
+
+def chung_lu_weights(n: int, gamma: float, mean_deg: float) -> np.ndarray:
+    """Expected degrees w_i = c (i + i0)^(-1/(gamma-1)) with mean `mean_deg` and the largest equal to
+    sqrt(n * mean_deg) (the usual cap that keeps every pair probability <= 1)."""
+    beta = 1.0 / (gamma - 1.0)
+    idx = np.arange(n, dtype=np.float64)
+    target = math.sqrt(n * mean_deg)
+
+    def wmax(i0):
+        s = np.sum((idx + i0) ** -beta)
+        return mean_deg * n / s * i0 ** -beta
+
+    lo, hi = 1e-3, float(n)
+    for _ in range(200):
+        mid = math.sqrt(lo * hi)
+        if wmax(mid) > target:
+            lo = mid
+        else:
+            hi = mid
+    i0 = math.sqrt(lo * hi)
+    w = (idx + i0) ** -beta
+    return w * (mean_deg * n / w.sum())
+
+
+def chung_lu_candidates(n: int, count: int, gamma: float, mean_deg: float, g: torch.Generator, device) -> torch.Tensor:
+    w = torch.from_numpy(chung_lu_weights(n, gamma, mean_deg)).to(device)
+    cdf = torch.cumsum(w, 0)
+    cdf = cdf / cdf[-1]
+    perm = torch.randperm(n, generator=g, device=device)  # spread the hubs over the id space
+    out = torch.empty(count, dtype=torch.int64, device=device)
+    for s in range(0, count, _CHUNK):
+        m = min(_CHUNK, count - s)
+        u = torch.searchsorted(cdf, torch.rand(m, generator=g, device=device, dtype=torch.float64)).clamp_(max=n - 1)
+        v = torch.searchsorted(cdf, torch.rand(m, generator=g, device=device, dtype=torch.float64)).clamp_(max=n - 1)
+        u, v = perm[u], perm[v]
+        lo, hi = torch.minimum(u, v), torch.maximum(u, v)
+        key = (lo << 32) | hi
+        key[u == v] = -1
+        out[s:s + m] = key
+    return out
+
+
+def first_occurrences(keys: torch.Tensor) -> torch.Tensor:
+    """Keys with self loops (-1) and repeats removed, first occurrence kept, draw order kept."""
+    sk, si = torch.sort(keys, stable=True)
+    first = torch.ones_like(sk, dtype=torch.bool)
+    first[1:] = sk[1:] != sk[:-1]
+    first &= sk >= 0
+    keep = si[first]
+    del sk, si, first
+    keep, _ = torch.sort(keep)
+    return keys[keep]
+
+
+def typed_stream(ins_keys: torch.Tensor, n_del: int, g: torch.Generator, want_final: bool = False):
+    """Inserts in order plus n_del deletes of distinct inserted edges, each scheduled uniformly
+    between its insert and the end.  Returns (pairs int32 (N,2), types uint8 (N,), final keys or None)."""
+    device = ins_keys.device
+    n_ins = ins_keys.numel()
+    n_del = min(n_del, n_ins)
+    dj = torch.randperm(n_ins, generator=g, device=device)[:n_del]
+    u = torch.rand(n_del, generator=g, device=device, dtype=torch.float64)
+    dkey = dj.to(torch.float64) + 0.5 + u * (n_ins - dj.to(torch.float64))
+    ikey = torch.arange(n_ins, device=device, dtype=torch.float64)
+    order = torch.sort(torch.cat([ikey, dkey]), stable=True).indices
+    del ikey, dkey, u
+    which = torch.cat([torch.arange(n_ins, device=device), dj])[order]
+    types = (order >= n_ins).to(torch.uint8)
+    del order
+    keys = ins_keys[which]
+    swap = torch.rand(n_ins, generator=g, device=device) < 0.5  # orientation per edge
+    sw = swap[which]
+    lo = (keys >> 32).to(torch.int32)
+    hi = (keys & 0xFFFFFFFF).to(torch.int32)
+    pairs = torch.stack([torch.where(sw, hi, lo), torch.where(sw, lo, hi)], dim=1).contiguous()
+    final = None
+    if want_final:
+        alive = torch.ones(n_ins, dtype=torch.bool, device=device)
+        alive[dj] = False
+        final = ins_keys[alive]
+    return pairs, types, final
+
+
+def _inserts(cand_fn, n_ins: int, g, device) -> torch.Tensor:
+    draw = int(n_ins * 1.15) + 1024
+    keys = first_occurrences(cand_fn(draw))
+    while keys.numel() < n_ins:  # top up with more draws (keeps the first occurrence rule)
+        more = cand_fn(max(1024, int((n_ins - keys.numel()) * 2)))
+        keys = first_occurrences(torch.cat([keys, more]))
+    return keys[:n_ins].contiguous()
+
+
+def rmat_stream(scale: int, n_updates: int, del_frac: float, seed: int, device="cuda", a=0.57, b=0.19, c=0.19,
+                want_final: bool = False):
+    """R-MAT(a, b, c, d = 1-a-b-c) stream on 2^scale vertices with del_frac of the updates deleting
+    present edges."""
+    g = _gen(seed, device)
+    n_del = int(round(n_updates * del_frac))
+    n_ins = n_updates - n_del
+    keys = _inserts(lambda cnt: rmat_candidates(scale, cnt, a, b, c, g, device), n_ins, g, device)
+    return typed_stream(keys, n_del, g, want_final)
+
+
+def chung_lu_stream(n: int, n_updates: int, del_frac: float, seed: int, device="cuda", gamma=2.1, mean_deg=64.0,
+                    want_final: bool = False):
+    """Chung-Lu stream: endpoints drawn independently with probability proportional to power-law
+    expected degrees (exponent `gamma`, mean `mean_deg`, capped at sqrt(n * mean_deg))."""
+    g = _gen(seed, device)
+    n_del = int(round(n_updates * del_frac))
+    n_ins = n_updates - n_del
+    keys = _inserts(lambda cnt: chung_lu_candidates(n, cnt, gamma, mean_deg, g, device), n_ins, g, device)
+    return typed_stream(keys, n_del, g, want_final)
+
+
+def keys_to_pairs(keys: torch.Tensor) -> torch.Tensor:
+    return torch.stack([(keys >> 32).to(torch.int32), (keys & 0xFFFFFFFF).to(torch.int32)], dim=1).contiguous()
+
+
+def exact_components(n: int, edges: np.ndarray) -> tuple[int, np.ndarray]:
+    """Connected components of an edge list with scipy (independent of this library); labels are
+    the smallest vertex id of each component."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    m = sp.coo_matrix((np.ones(e.shape[0], dtype=np.int8), (e[:, 0], e[:, 1])), shape=(n, n)).tocsr()
+    ncc, lab = connected_components(m, directed=False)
+    first = np.full(ncc, n, dtype=np.int64)
+    np.minimum.at(first, lab, np.arange(n))
+    return int(ncc), first[lab].astype(np.uint32)

@@ -142,9 +142,31 @@ int gz_alg_construct_from_serialized_data(const char* path, const gz_alg_config*
 /* Packed byte image of `count` vertex sketches starting at v0 (what write_binary emits per vertex,
  * Sketch::serialize): count * gz_alg_sketch_bytes() bytes.  Applies buffered updates first. */
 int gz_alg_dump_sketches(gz_alg* a, uint32_t v0, uint32_t count, void* out, size_t out_bytes);
+/* Order-sensitive 64-bit digest of every vertex's bucket array (no reference counterpart: a cheap
+ * whole-state check).  With pos = v * get_buckets() + b over the packed image,
+ *   digest = sum_{v,b} fmix64(x + pos),  x = alpha ^ fmix64(gamma ^ pos * 0x9E3779B97F4A7C15)
+ * (mod 2^64, fmix64 = MurmurHash3's 64-bit finaliser).  Applies buffered updates first. */
+int gz_alg_digest(gz_alg* a, uint64_t* out);
 /* time hooks / counters: last_query_rounds (include/cc_sketch_alg.h:252), updates applied */
 size_t gz_alg_last_query_rounds(const gz_alg* a);
 uint64_t gz_alg_get_total_updates(const gz_alg* a);
+/* Kernel timing (no reference counterpart; the reference only has steady_clock hooks,
+ * include/cc_sketch_alg.h:250-252).  When enabled, HIP events on the instance's stream bracket every
+ * sketch-update launch and every Borůvka round; alg_bytes is the algorithmic HBM traffic of those
+ * launches: per vertex batch of k destinations 4k + 2*min(S, k*(C+1)*B) with S the sketch bytes,
+ * C the columns and B the bucket bytes; per Borůvka round n*(D+1)*B + 8n. */
+typedef struct {
+  uint64_t launches;
+  double total_ms;
+  double alg_bytes;
+  uint64_t items; /* directed updates (update kernel) or rounds (Borůvka) */
+} gz_kernel_stat;
+typedef struct {
+  gz_kernel_stat update;
+  gz_kernel_stat boruvka;
+} gz_alg_profile;
+int gz_alg_set_profiling(gz_alg* a, int enable);
+int gz_alg_get_profile(gz_alg* a, gz_alg_profile* out);
 /* Device footprint in bytes of an instance with these parameters (sketch slab + work buffers). */
 uint64_t gz_alg_footprint_bytes(uint32_t num_vertices, const gz_alg_config* cfg);
 

@@ -34,6 +34,7 @@ def test_library_exports_every_declared_symbol(header):
 
 def test_binding_covers_the_header():
     names = set(_declared("gz_hip.h"))
+    names |= set(_declared("gz_serving.h"))
     assert names <= set(_lib.exported_symbols()) | set(_lib._EXTRA_SIGS)
 
 
