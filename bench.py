@@ -87,7 +87,7 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--flush-threshold", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=8_000_000, help="updates in the CPU-baseline prefix")
-    ap.add_argument("--plain-sample", type=int, default=20_000_000, help="updates in the plain-form prefix")
+    ap.add_argument("--plain-sample", type=int, default=4_000_000, help="updates in the plain-form prefix")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()

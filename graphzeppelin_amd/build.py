@@ -69,5 +69,21 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_cpp_tests(verbose: bool = False) -> str:
+    """Compile tests/cpp/driver_test.cpp (reference-shaped C++ tests of the facade in
+    include/graphzeppelin_amd.hpp) against libgz_hip.so."""
+    src = os.path.join(REPO, "tests", "cpp", "driver_test.cpp")
+    out = os.path.join(REPO, "tests", "cpp", "driver_test")
+    cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-I", os.path.join(REPO, "include"), src, "-o", out,
+           "-L", HERE, "-lgz_hip", "-Wl,-rpath,$ORIGIN/../../graphzeppelin_amd", "-lpthread"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("C++ facade test build failed\n" + res.stdout + res.stderr)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_tests(verbose=True)
