@@ -154,19 +154,19 @@ __device__ inline int wave_sample(uint64_t det_a, G det_g, uint64_t a, G gm, boo
 }
 
 // Act on a supernode's sample (sample_supernode, src/cc_sketch_alg.cpp:137-166).  Lane 0 only.
-__device__ inline void act_on_sample(const DsuDev& d, int code, uint64_t idx) {
-  if (code == kFail) { st_rlx(d.flags + kFlagModified, 1u); return; }
+__device__ inline void act_on_sample(const DsuDev& d, int code, uint64_t idx, uint32_t* modified) {
+  if (code == kFail) { st_rlx(modified, 1u); return; }
   if (code != kGood) return;
   uint32_t u = (uint32_t)(idx >> 32), v = (uint32_t)idx;
   if (u >= d.n || v >= d.n) {
     // index decoded from a checksum collision; counted as a failed sample
-    st_rlx(d.flags + kFlagModified, 1u);
+    st_rlx(modified, 1u);
     st_rlx(d.flags + kFlagBad, 1u);
     return;
   }
   if (dsu_merge(d.parent, u, v, d.prio_seed)) {
     sf_record(d, u, v);
-    st_rlx(d.flags + kFlagModified, 1u);
+    st_rlx(modified, 1u);
   }
 }
 
@@ -459,8 +459,14 @@ __global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, 
 // ================================================================================================
 // Borůvka (boruvka_emulation, src/cc_sketch_alg.cpp:464-513)
 // ================================================================================================
-__global__ void k_dsu_reset(uint32_t* parent, uint32_t n) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) parent[i] = i;
+__global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uint64_t sf_cap, uint32_t* flags,
+                            uint32_t* mod, uint32_t nmod) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (size_t i = t; i < n; i += stride) parent[i] = (uint32_t)i;
+  for (size_t i = t; i < sf_cap; i += stride) sf_hash[i] = kEmpty;
+  if (t < kNumFlags) flags[t] = 0;
+  for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
 }
 
 // Round 0 (run_round_zero, 223-244): every vertex samples column 0 of its own sketch.  One wave
@@ -479,23 +485,31 @@ k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma,
     const G gm = valid ? gamma[base + lane] : 0;
     uint64_t idx;
     const int code = wave_sample<G>(det_a, det_g, a, gm, valid, g.chk_bf, &idx);
-    if (lane == 0) act_on_sample(d, code, idx);
+    if (lane == 0) act_on_sample(d, code, idx, d.mod);
   }
 }
 
-// Root of every vertex (create_merge_instructions' find_root pass, 382-403)
-__global__ void k_compress(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-    root[v] = dsu_find(parent, v);
-}
-
-// Wave-aggregated counting sort by root: lanes that share a root issue one atomic together.
-__global__ void k_count_roots(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cnt) {
+// ---- rounds >= 1 (perform_boruvka_round 246-380 + create_merge_instructions 382-462) ----------
+// Every kernel of round r first reads mod[r-1]: when round r-1 changed nothing the whole round is
+// a no-op, which lets the host queue several rounds before it looks at the flags.
+//
+// Regrouping by root: A) root of every vertex + wave-aggregated count per root; B) each root gets
+// a contiguous segment [off, off+cnt) from one running cursor (segments are contiguous but not in
+// root order, which Borůvka does not need); C) scatter the vertices into their segments.
+__global__ void k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root,
+                                uint32_t* __restrict__ cnt, uint32_t* __restrict__ seg_cursor,
+                                const uint32_t* __restrict__ prev_mod) {
+  if (!ld_acq(prev_mod)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *seg_cursor = 0;
   const uint32_t lane = lane_id();
   for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
     const uint32_t v = v0 + lane;
     const bool active = v < n;
-    const uint32_t r = active ? root[v] : 0;
+    uint32_t r = 0;
+    if (active) {
+      r = dsu_find(parent, v);
+      root[v] = r;
+    }
     uint64_t todo = __ballot(active);
     while (todo) {
       const int leader = __builtin_ctzll(todo);
@@ -506,8 +520,28 @@ __global__ void k_count_roots(const uint32_t* __restrict__ root, uint32_t n, uin
     }
   }
 }
-__global__ void k_scatter_roots(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
-                                uint32_t* __restrict__ members) {
+__global__ void k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off,
+                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ seg_cursor,
+                           const uint32_t* __restrict__ prev_mod) {
+  if (!ld_acq(prev_mod)) return;
+  const uint32_t lane = lane_id();
+  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
+    const uint32_t v = v0 + lane;
+    const uint32_t c = v < n ? cnt[v] : 0;
+    const uint64_t incl = wave_incl_scan_u64(c);
+    uint32_t base = 0;
+    if (lane == kWave - 1 && incl) base = atomicAdd(seg_cursor, (uint32_t)incl);
+    base = rd_lane_u32(base, kWave - 1);
+    if (c) {
+      const uint32_t o = base + (uint32_t)(incl - c);
+      off[v] = o;
+      cursor[v] = o;
+    }
+  }
+}
+__global__ void k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
+                             uint32_t* __restrict__ members, const uint32_t* __restrict__ prev_mod) {
+  if (!ld_acq(prev_mod)) return;
   const uint32_t lane = lane_id();
   const uint64_t lt = (1ull << lane) - 1;
   for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
@@ -528,84 +562,103 @@ __global__ void k_scatter_roots(const uint32_t* __restrict__ root, uint32_t n, u
   }
 }
 
-// Rounds >= 1 (perform_boruvka_round, 246-380, with range_merge, src/sketch.cpp:156-179):
-// the members array is grouped by root; each wave owns kWave consecutive positions.  Lane l < D
-// accumulates depth l of column `round`, lane D the deterministic bucket.  A supernode that lies
-// inside one wave's range is sampled straight from registers; one that spans several ranges is
-// XOR-ed into an accumulator slot with device atomics and sampled by the last wave to arrive.
-template <class G>
-__device__ inline void finalize_segment(uint32_t R, uint64_t acc_a, G acc_g, uint32_t p0, uint32_t p1,
-                                        const uint32_t* off, const uint32_t* cnt, const Geo& g, const DsuDev& d,
-                                        uint64_t* slot_a, uint64_t* slot_g, uint32_t* done) {
-  const uint32_t lane = lane_id();
-  const uint32_t s = off[R], e = s + cnt[R];
-  const bool valid = lane < g.D;
-  if (s >= p0 && e <= p1) {
-    const uint64_t det_a = rd_lane_u64(acc_a, g.D);
-    const G det_g = (G)rd_lane_u64((uint64_t)acc_g, g.D);
-    uint64_t idx;
-    const int code = wave_sample<G>(det_a, det_g, acc_a, acc_g, valid, g.chk_bf, &idx);
-    if (lane == 0) act_on_sample(d, code, idx);
-    return;
-  }
-  const uint32_t first = s / kWave, last = (e - 1) / kWave, parts = last - first + 1;
-  uint64_t* sa = slot_a + (size_t)first * kWave;
-  uint64_t* sg = slot_g + (size_t)first * kWave;
-  if (lane <= g.D) {
-    if (acc_a) gbl_xor(sa + lane, acc_a);
-    if (acc_g) gbl_xor(sg + lane, (uint64_t)acc_g);
-  }
-  __threadfence();
-  uint32_t old = 0;
-  if (lane == 0) old = atomicAdd(done + first, 1u);
-  old = rd_lane_u32(old, 0);
-  if (old != parts - 1) return;
-  __threadfence();
-  const uint64_t ta = lane <= g.D ? ld_acq(sa + lane) : 0;
-  const G tg = lane <= g.D ? (G)ld_acq(sg + lane) : 0;
-  const uint64_t det_a = rd_lane_u64(ta, g.D);
-  const G det_g = (G)rd_lane_u64((uint64_t)tg, g.D);
-  uint64_t idx;
-  const int code = wave_sample<G>(det_a, det_g, ta, tg, valid, g.chk_bf, &idx);
-  if (lane == 0) act_on_sample(d, code, idx);
-}
+// Segmented XOR-reduce + sample, one level (range_merge of column `round` + the deterministic
+// bucket over every member of a supernode, then sample_supernode).  Entries are grouped by root;
+// a wave owns 64 consecutive entries, lane l < D accumulates depth l of the column and lane D the
+// deterministic bucket.  A supernode whose entries all lie inside the wave is sampled from
+// registers.  A supernode that crosses the wave's first (head) or last (tail) boundary leaves its
+// partial XOR in out[2w] / out[2w+1] under its root; unused outputs get root kNoRoot and zeros.
+// The next level reduces those 2*waves entries the same way, so a supernode spanning many waves is
+// combined without any atomics; the last level has a single wave.
+constexpr uint32_t kNoRoot = 0xFFFFFFFFu;
 
-template <class G>
+template <class G, bool FIRST>
 __global__ void __launch_bounds__(256)
-k_reduce_sample(const uint32_t* __restrict__ members, const uint32_t* __restrict__ root,
-                const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t round,
-                const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, DsuDev d,
-                uint64_t* __restrict__ slot_a, uint64_t* __restrict__ slot_g, uint32_t* __restrict__ done) {
+k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
+            const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint32_t round,
+            const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
+            uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, DsuDev d,
+            const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod, uint32_t* __restrict__ zero_cnt) {
+  if (!ld_acq(prev_mod)) return;
   const uint32_t lane = lane_id();
+  if (FIRST && zero_cnt) {  // clear the count buffer the next round will use
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) zero_cnt[i] = 0;
+  }
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t p0 = wave * kWave;
-  if (p0 >= g.n) return;
-  const uint32_t p1 = (p0 + kWave < g.n) ? p0 + kWave : g.n;
+  if (p0 >= n_entries) return;
+  const uint32_t p1 = (p0 + kWave < n_entries) ? p0 + kWave : n_entries;
   const uint32_t len = p1 - p0;
-  const uint32_t m = lane < len ? members[p0 + lane] : 0;
-  const uint32_t rt = lane < len ? root[m] : 0;
-  // lane -> bucket offset within the vertex: depth `lane` of column `round`, or the det bucket
-  const bool in_col = lane < g.D;
-  const bool is_det = lane == g.D;
-  const uint32_t boff = in_col ? round * g.D + lane : (g.nb - 1);
-  const bool loads = in_col || is_det;
 
-  uint64_t acc_a = 0;
-  G acc_g = 0;
-  uint32_t cur = rd_lane_u32(rt, 0);
+  auto entry_root = [&](uint32_t p) -> uint32_t { return FIRST ? vroot[members[p]] : in_root[p]; };
+  uint32_t m = 0, rt = kNoRoot;
+  if (lane < len) {
+    if (FIRST) {
+      m = members[p0 + lane];
+      rt = vroot[m];
+    } else {
+      rt = in_root[p0 + lane];
+    }
+  }
+  const uint32_t first_root = rd_lane_u32(rt, 0), last_root = rd_lane_u32(rt, len - 1);
+  const bool head_open = p0 > 0 && first_root != kNoRoot && entry_root(p0 - 1) == first_root;
+  const bool tail_open = p1 < n_entries && last_root != kNoRoot && entry_root(p1) == last_root;
+
+  const bool in_col = lane < g.D;
+  const uint32_t boff = in_col ? round * g.D + lane : (g.nb - 1);
+  const bool loads = lane <= g.D;
+  uint64_t* out_a = out_part + (size_t)(2 * wave) * 2 * kWave;  // entry layout: alpha[64] then gamma[64]
+  bool wrote_head = false, wrote_tail = false;
+
+  auto emit = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g, uint32_t seg_start, uint32_t seg_end) {
+    const bool started_before = seg_start == p0 && head_open;
+    const bool continues = seg_end == p1 && tail_open;
+    if (!started_before && !continues) {
+      if (R == kNoRoot) return;
+      const uint64_t det_a = rd_lane_u64(acc_a, g.D);
+      const G det_g = (G)rd_lane_u64(acc_g, g.D);
+      uint64_t idx;
+      const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
+      if (lane == 0) act_on_sample(d, code, idx, cur_mod);
+      return;
+    }
+    // Partial of a spanning supernode: the head slot if it started before this wave, else the tail
+    // slot.  When it covers the whole wave the tail slot carries the same root with a zero value,
+    // so the supernode's entries stay contiguous at the next level.
+    const uint32_t slot = started_before ? 0 : 1;
+    uint64_t* o = out_a + (size_t)slot * 2 * kWave;
+    o[lane] = acc_a;
+    o[kWave + lane] = acc_g;
+    if (lane == 0) out_root[2 * wave + slot] = R;
+    if (slot == 0) wrote_head = true;
+    else wrote_tail = true;
+    if (started_before && continues) {
+      uint64_t* t = out_a + 2 * kWave;
+      t[lane] = 0;
+      t[kWave + lane] = 0;
+      if (lane == 0) out_root[2 * wave + 1] = R;
+      wrote_tail = true;
+    }
+  };
+
+  uint64_t acc_a = 0, acc_g = 0;
+  uint32_t cur = first_root, seg_start = p0;
   constexpr uint32_t U = 4;
   for (uint32_t j = 0; j < len; j += U) {
-    uint64_t va[U];
-    G vg[U];
+    uint64_t va[U], vg[U];
 #pragma unroll
     for (uint32_t k = 0; k < U; ++k) {
       va[k] = 0;
       vg[k] = 0;
-      if (j + k < len) {
-        const uint32_t v = rd_lane_u32(m, j + k);
-        if (loads) {
-          va[k] = alpha[(size_t)v * g.stride + boff];
-          vg[k] = gamma[(size_t)v * g.stride + boff];
+      if (j + k < len && loads) {
+        if (FIRST) {
+          const size_t o = (size_t)rd_lane_u32(m, j + k) * g.stride + boff;
+          va[k] = alpha[o];
+          vg[k] = (uint64_t)gamma[o];
+        } else {
+          const uint64_t* e = in_part + (size_t)(p0 + j + k) * 2 * kWave;
+          va[k] = e[lane];
+          vg[k] = e[kWave + lane];
         }
       }
     }
@@ -614,17 +667,30 @@ k_reduce_sample(const uint32_t* __restrict__ members, const uint32_t* __restrict
       if (j + k < len) {
         const uint32_t R = rd_lane_u32(rt, j + k);
         if (R != cur) {
-          finalize_segment<G>(cur, acc_a, acc_g, p0, p1, off, cnt, g, d, slot_a, slot_g, done);
+          emit(cur, acc_a, acc_g, seg_start, p0 + j + k);
           acc_a = 0;
           acc_g = 0;
           cur = R;
+          seg_start = p0 + j + k;
         }
         acc_a ^= va[k];
         acc_g ^= vg[k];
       }
     }
   }
-  finalize_segment<G>(cur, acc_a, acc_g, p0, p1, off, cnt, g, d, slot_a, slot_g, done);
+  emit(cur, acc_a, acc_g, seg_start, p1);
+  // unused output slots carry no root and no value
+  if (!wrote_head) {
+    out_a[lane] = 0;
+    out_a[kWave + lane] = 0;
+    if (lane == 0) out_root[2 * wave] = kNoRoot;
+  }
+  if (!wrote_tail) {
+    uint64_t* o = out_a + 2 * kWave;
+    o[lane] = 0;
+    o[kWave + lane] = 0;
+    if (lane == 0) out_root[2 * wave + 1] = kNoRoot;
+  }
 }
 
 // ConnectedComponents (src/return_types.cpp:5-19): root of every vertex and the root count.
@@ -863,9 +929,9 @@ void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const Ds
 }
 
 void launch_dsu_reset(const DsuDev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_dsu_reset, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.parent, d.n);
-  launch_fill_u64(d.sf_hash, kEmpty, d.sf_mask + 1, s);
-  launch_fill_u32(d.flags, 0, kNumFlags, s);
+  const size_t work = std::max<size_t>(d.n, d.sf_mask + 1);
+  hipLaunchKernelGGL(k_dsu_reset, dim3(grid_for(work, 256)), dim3(256), 0, s, d.parent, d.n, d.sf_hash,
+                     d.sf_mask + 1, d.flags, d.mod, d.nmod);
 }
 
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
@@ -877,26 +943,55 @@ void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, 
     hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, d);
 }
 
+size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) {
+  size_t L = 0;
+  uint32_t e = n;
+  while (true) {
+    if (L < max_levels && entries_per_level) entries_per_level[L] = e;
+    ++L;
+    const uint32_t waves = (e + kWave - 1) / kWave;
+    if (waves <= 1) break;
+    e = 2 * waves;
+  }
+  return L;
+}
+
 void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, hipStream_t s) {
   const uint32_t n = g.n;
-  const uint32_t nslots = n / kWave + 1;
-  hipLaunchKernelGGL(k_compress, dim3(grid_for(n, 256)), dim3(256), 0, s, d.parent, n, rb.root);
-  launch_fill_u32(rb.cnt, 0, n, s);
-  hipLaunchKernelGGL(k_count_roots, dim3(grid_for(n, 256)), dim3(256), 0, s, rb.root, n, rb.cnt);
-  scan_counts(rb.cnt, n, 0, rb.partials, rb.off, nullptr, rb.cursor, rb.totals, s);
-  hipLaunchKernelGGL(k_scatter_roots, dim3(grid_for(n, 256)), dim3(256), 0, s, rb.root, n, rb.cursor, rb.members);
-  launch_fill_u64(rb.acc_a, 0, (size_t)nslots * kWave, s);
-  launch_fill_u64(rb.acc_g, 0, (size_t)nslots * kWave, s);
-  launch_fill_u32(rb.done, 0, nslots, s);
-  launch_fill_u32(d.flags + kFlagModified, 0, 1, s);
-  const uint32_t grid = (uint32_t)(((size_t)((n + kWave - 1) / kWave) * kWave + 255) / 256);
-  if (gbytes == 4)
-    hipLaunchKernelGGL(k_reduce_sample<uint32_t>, dim3(grid), dim3(256), 0, s, rb.members, rb.root, rb.off, rb.cnt,
-                       round, alpha, (const uint32_t*)gamma, g, d, rb.acc_a, rb.acc_g, rb.done);
-  else
-    hipLaunchKernelGGL(k_reduce_sample<uint64_t>, dim3(grid), dim3(256), 0, s, rb.members, rb.root, rb.off, rb.cnt,
-                       round, alpha, (const uint64_t*)gamma, g, d, rb.acc_a, rb.acc_g, rb.done);
+  const uint32_t* prev = d.mod + (round - 1);
+  uint32_t* cur = d.mod + round;
+  uint32_t* cnt = rb.cnt;  // zero on entry; the level-1 reduce clears it again for the next round
+  const uint32_t grid = grid_for(n, 256);
+  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(256), 0, s, d.parent, n, rb.root, cnt, rb.totals, prev);
+  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(256), 0, s, cnt, n, rb.off, rb.cursor, rb.totals, prev);
+  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(256), 0, s, rb.root, n, rb.cursor, rb.members, prev);
+  uint32_t ent[32];
+  const size_t L = round_levels(n, ent, 32);
+  for (size_t l = 0; l < L; ++l) {
+    const uint32_t e = ent[l];
+    const uint32_t waves = (e + kWave - 1) / kWave;
+    const uint32_t blocks = (waves * kWave + 255) / 256;
+    uint64_t* in_p = rb.part[(l + 1) & 1];
+    uint32_t* in_r = rb.proot[(l + 1) & 1];
+    uint64_t* out_p = rb.part[l & 1];
+    uint32_t* out_r = rb.proot[l & 1];
+    if (l == 0) {
+      if (gbytes == 4)
+        hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
+                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, cnt);
+      else
+        hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
+                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, cnt);
+    } else {
+      if (gbytes == 4)
+        hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
+                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, nullptr);
+      else
+        hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
+                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, nullptr);
+    }
+  }
 }
 
 void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s) {

@@ -13,6 +13,8 @@ struct DsuDev {
   uint64_t sf_mask;
   uint32_t* sf_list;   // 2 * (n - 1) endpoints, appended in merge order
   uint32_t* flags;     // FlagSlot entries
+  uint32_t* mod;       // per-round "modified" flags, mod[r] for round r (nmod entries)
+  uint32_t nmod;
   uint32_t prio_seed;
   uint32_t n;
 };
@@ -65,17 +67,18 @@ void launch_dsu_reset(const DsuDev& d, hipStream_t s);
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
                           hipStream_t s);
 struct RoundBufs {
-  uint32_t* root;      // n
-  uint32_t* cnt;       // n
-  uint32_t* off;       // n + 1
-  uint32_t* cursor;    // n
-  uint32_t* members;   // n
-  uint64_t* partials;  // scan partials
-  uint32_t* totals;    // scratch for the scan
-  uint64_t* acc_a;     // (n/64 + 1) * 64
-  uint64_t* acc_g;     // (n/64 + 1) * 64 (u32 or u64 entries stored as u64)
-  uint32_t* done;      // n/64 + 1
+  uint32_t* root;      // n      root of every vertex this round
+  uint32_t* cnt;       // n      members per root (zero between rounds)
+  uint32_t* off;       // n      segment start per root
+  uint32_t* cursor;    // n      scatter cursor per root
+  uint32_t* members;   // n      vertices grouped by root
+  uint32_t* totals;    // [0]    segment allocation cursor
+  uint64_t* part[2];   // partial-XOR entries of the multi-level reduce, 2*ceil(n/64) entries of 128 u64
+  uint32_t* proot[2];  // root of each partial entry
 };
+// Number of reduce levels for n entries (entries_per_level gets each level's entry count).
+size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
+// Queue round `round` (>= 1); every kernel of it is a no-op when d.mod[round - 1] == 0.
 void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, hipStream_t s);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);

@@ -208,24 +208,30 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   sf_hash.alloc(cap);
   sf_list.alloc(2ull * n);
   flags.alloc(kNumFlags);
+  mod.alloc(geo.samples + 2);
   labels.alloc(n + 1);
   dsu.parent = parent.p;
   dsu.sf_hash = sf_hash.p;
   dsu.sf_mask = cap - 1;
   dsu.sf_list = sf_list.p;
   dsu.flags = flags.p;
+  dsu.mod = mod.p;
+  dsu.nmod = geo.samples + 2;
   dsu.prio_seed = (uint32_t)(seed * 0x9E3779B97F4A7C15ULL >> 32);
   dsu.n = n;
   launch_dsu_reset(dsu, st);
 
   // Borůvka round buffers
-  const uint32_t nslots = n / 64 + 1;
-  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n + 1); r_cursor.alloc(n); r_members.alloc(n);
-  r_totals.alloc(4); r_done.alloc(nslots);
-  r_partials.alloc(group_partials_len(n));
-  r_acc_a.alloc((size_t)nslots * 64); r_acc_g.alloc((size_t)nslots * 64);
-  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, r_members.p, r_partials.p, r_totals.p,
-                 r_acc_a.p, r_acc_g.p, r_done.p};
+  const uint32_t waves1 = (n + 63) / 64;
+  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_members.alloc(n);
+  r_totals.alloc(4);
+  for (int i = 0; i < 2; ++i) {
+    r_part[i].alloc((size_t)(2 * waves1 + 2) * 128);
+    r_proot[i].alloc(2 * waves1 + 2);
+  }
+  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, r_members.p, r_totals.p, {r_part[0].p, r_part[1].p},
+                 {r_proot[0].p, r_proot[1].p}};
+  launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
   cap_pairs = cfg.flush_threshold ? (size_t)cfg.flush_threshold : default_cap_pairs(n, desired_updates_per_batch());
@@ -246,7 +252,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   vstamp.assign(n, 0);
   vtiles.assign(n, 0);
   eager_cap = size_t(1) << 20;
-  hflags.alloc(kNumFlags);
+  hflags.alloc(std::max<size_t>(kNumFlags, geo.samples + 2));
   hsmall.alloc(16);
   GZ_HIP(hipEventCreateWithFlags(&eager_ev, hipEventDisableTiming));
   GZ_HIP(hipStreamSynchronize(st));
@@ -568,29 +574,45 @@ void GzAlg::ensure_answer() {
   boruvka();
 }
 
-// boruvka_emulation, src/cc_sketch_alg.cpp:464-513
+// boruvka_emulation, src/cc_sketch_alg.cpp:464-513.  Rounds are queued in groups; every kernel
+// of round r is a no-op once round r-1 changed nothing, so the host reads the per-round flags once
+// per group instead of once per round.
 void GzAlg::boruvka() {
   update_locked = true;
   try {
     const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
     launch_dsu_reset(dsu, st);
-    if (prof) prof_begin(1, round_bytes, false, 1);
+    launch_fill_u32(r_cnt.p, 0, n, st);
+    if (prof) prof_begin(1, 0, false, 0);
     launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, st);
-    if (prof) prof_end();
     GZ_HIP(hipGetLastError());
-    uint32_t round = 0;
-    bool modified = read_flag(kFlagModified) != 0;
-    while (modified) {
-      ++round;
-      if (round >= geo.samples)
-        throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
-      if (prof) prof_begin(1, round_bytes, false, 1);
-      launch_round(gb, round, alpha.p, gamma.p, geo, dsu, rb, st);
-      if (prof) prof_end();
+    const uint32_t last_round = geo.samples - 1;  // round index `samples` would be out of samples
+    uint32_t next = 1, group = 2;
+    int64_t rounds = -1;
+    while (true) {
+      const uint32_t hi = std::min(next + group - 1, last_round);
+      for (uint32_t r = next; r <= hi && next <= last_round; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, st);
       GZ_HIP(hipGetLastError());
-      modified = read_flag(kFlagModified) != 0;
+      const uint32_t upto = std::max(hi, next - 1);
+      GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)(upto + 1) * 4, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipStreamSynchronize(st));
+      for (uint32_t r = 0; r <= upto; ++r)
+        if (!hflags.p[r]) {
+          rounds = r;
+          break;
+        }
+      if (rounds >= 0) break;
+      if (upto >= last_round)
+        throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
+      next = upto + 1;
+      group = 4;
     }
-    last_rounds = round;
+    if (prof) {
+      prof_end();
+      timed.back().host_bytes = round_bytes * (double)(rounds + 1);
+      timed.back().items = (uint64_t)rounds + 1;
+    }
+    last_rounds = (size_t)rounds;
     dsu_valid = true;
     if (prof) prof_collect();
   } catch (...) {

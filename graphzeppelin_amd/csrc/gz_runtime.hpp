@@ -140,8 +140,9 @@ class GzAlg {
   DevBuf<uint8_t> qout;
   DsuDev dsu{};
   // Borůvka round buffers
-  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members, r_totals, r_done;
-  DevBuf<uint64_t> r_partials, r_acc_a, r_acc_g;
+  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members, r_totals, mod;
+  DevBuf<uint64_t> r_part[2];
+  DevBuf<uint32_t> r_proot[2];
   RoundBufs rb{};
   // device stream ingest (pending pairs -> grouping -> tiles)
   size_t cap_pairs;
@@ -176,7 +177,7 @@ class GzAlg {
   DevBuf<uint8_t> eager_merged;
   hipEvent_t eager_ev = nullptr;
   size_t eager_used = 0, eager_cap;
-  PinBuf<uint32_t> hflags;
+  PinBuf<uint32_t> hflags;  // >= kNumFlags and >= samples + 2 entries
   PinBuf<uint32_t> hsmall;
 
   // kernel timing (gz_alg_set_profiling)
