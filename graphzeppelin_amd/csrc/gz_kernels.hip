@@ -202,24 +202,25 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t C = g.C, D = g.D, Dp = g.Dp;
   const uint32_t lds_n = C * Dp + 1;  // + deterministic bucket at C*Dp
+  const uint32_t nb = C * D + 1;
   uint64_t* s_bf = reinterpret_cast<uint64_t*>(smem);
   uint64_t* s_a = s_bf + C;
   G* s_g = reinterpret_cast<G*>(s_a + lds_n);
   __shared__ uint32_t s_tile;
 
   for (uint32_t i = threadIdx.x; i < C; i += BLOCK) s_bf[i] = col_bf[i];
+  // The delta is zeroed once here; afterwards the write-back clears every entry it reads, so the
+  // next tile starts from zero without a separate pass.  LDS entries with depth >= D (pitch
+  // padding) are never written.
+  for (uint32_t i = threadIdx.x; i < lds_n; i += BLOCK) { s_a[i] = 0; s_g[i] = 0; }
   const uint32_t ntiles = ntiles_dev ? *ntiles_dev : ntiles_host;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t start_col = lane % C;
   const uint64_t depth_cap = 1ull << D;
+  if (threadIdx.x == 0) s_tile = work_counter ? atomicAdd(work_counter, 1u) : blockIdx.x;
+  __syncthreads();
 
-  // work_counter == nullptr: static mapping, workgroup b processes tile b only
-  bool first = true;
   while (true) {
-    if (threadIdx.x == 0) s_tile = work_counter ? atomicAdd(work_counter, 1u) : (first ? blockIdx.x : ntiles);
-    first = false;
-    for (uint32_t i = threadIdx.x; i < lds_n; i += BLOCK) { s_a[i] = 0; s_g[i] = 0; }
-    __syncthreads();
     const uint32_t t = s_tile;
     if (t >= ntiles) break;
     const Tile tl = tiles[t];
@@ -251,27 +252,60 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
       lds_xor(s_g + C * Dp, det_g);
     }
     __syncthreads();
+    // prefetch the next tile while this one is written back (read after the barrier below)
+    if (threadIdx.x == 0) s_tile = work_counter ? atomicAdd(work_counter, 1u) : ntiles;
 
-    // write-back of the delta into the vertex's planes
+    // write-back: read-xor-write of the vertex rows, four buckets per thread in flight
     const size_t base = (size_t)tl.vertex * g.stride;
-    const uint32_t nb = C * D + 1;
-    for (uint32_t b = threadIdx.x; b < nb; b += BLOCK) {
-      uint32_t li;
-      if (b == C * D) {
-        li = C * Dp;
-      } else {
-        const uint32_t c = b / D;
-        li = c * Dp + (b - c * D);
+    for (uint32_t b0 = 0; b0 < nb; b0 += 4 * BLOCK) {
+      uint64_t da[4], ga[4];
+      G dg[4], gg[4];
+      uint32_t bi[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t b = b0 + k * BLOCK + threadIdx.x;
+        bi[k] = b;
+        da[k] = 0;
+        dg[k] = 0;
+        if (b < nb) {
+          uint32_t li;
+          if (b == nb - 1) {
+            li = C * Dp;
+          } else {
+            const uint32_t c = b / D;
+            li = c * Dp + (b - c * D);
+          }
+          da[k] = s_a[li];
+          dg[k] = s_g[li];
+          if ((da[k] | (uint64_t)dg[k]) != 0) {
+            s_a[li] = 0;
+            s_g[li] = 0;
+          }
+        }
       }
-      const uint64_t da = s_a[li];
-      const G dg = s_g[li];
-      if ((da | (uint64_t)dg) == 0) continue;
       if (tl.split) {
-        if (da) gbl_xor(alpha + base + b, da);
-        if (dg) gbl_xor(gamma + base + b, dg);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (da[k]) gbl_xor(alpha + base + bi[k], da[k]);
+          if (dg[k]) gbl_xor(gamma + base + bi[k], dg[k]);
+        }
       } else {
-        alpha[base + b] ^= da;
-        gamma[base + b] ^= dg;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ga[k] = 0;
+          gg[k] = 0;
+          if ((da[k] | (uint64_t)dg[k]) != 0) {
+            ga[k] = alpha[base + bi[k]];
+            gg[k] = gamma[base + bi[k]];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if ((da[k] | (uint64_t)dg[k]) != 0) {
+            alpha[base + bi[k]] = ga[k] ^ da[k];
+            gamma[base + bi[k]] = gg[k] ^ dg[k];
+          }
+        }
       }
     }
     __syncthreads();

@@ -455,6 +455,7 @@ void GzAlg::prof_collect() {
   if (timed.empty()) return;
   for (auto& t : timed) {
     float ms = 0;
+    GZ_HIP(hipEventSynchronize(t.b));
     GZ_HIP(hipEventElapsedTime(&ms, t.a, t.b));
     gz_kernel_stat& k = t.kind == 0 ? profile.update : profile.boruvka;
     k.launches += 1;

@@ -5,8 +5,8 @@
   1 = delete.
 * The ASCII graph format of test/res/multiples_graph_1024.txt: "n m" then "src dst" lines.
 * Generators emit type-consistent streams (no self loops, no insert of a present edge, deletes only
-  of present edges) as (src, dst) uint32 pairs plus a uint8 type array.  The small generators here
-  are numpy (PCG64 seeded); the large ones are native (graphzeppelin_amd/csrc/gz_streams.cpp).
+  of present edges) as (src, dst) uint32 pairs plus a uint8 type array.  The generators here are
+  numpy (PCG64 seeded); the 10^8-update ones run on the device with torch (graphzeppelin_amd/gen.py).
 """
 from __future__ import annotations
 
@@ -148,3 +148,60 @@ def grid_graph_edges(rows: int, cols: int) -> np.ndarray:
     h = np.stack([v[:, :-1].ravel(), v[:, 1:].ravel()], 1)
     w = np.stack([v[:-1, :].ravel(), v[1:, :].ravel()], 1)
     return np.concatenate([h, w]).astype(np.uint32)
+
+
+def _tri_pairs(s: int, k: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Linear index k in [0, s(s-1)/2) -> (i, j), i < j, row-major over the upper triangle."""
+    k = k.astype(np.float64)
+    i = (s - 2 - np.floor(np.sqrt(-8 * k + 4.0 * s * (s - 1) - 7) / 2 - 0.5)).astype(np.int64)
+    ki = k.astype(np.int64)
+    j = ki + i + 1 - s * (s - 1) // 2 + (s - i) * ((s - i) - 1) // 2
+    return i, j
+
+
+def planted_blocks_stream(n: int = 1 << 15, nblocks: int = 64, p: float = 0.05, churn: float = 0.3, seed: int = 2,
+                          small: tuple[int, int] = (16, 128)):
+    """Planted components: one giant block plus nblocks-1 blocks of uniform size in `small`, vertex
+    ids assigned by a seeded permutation; G(size, p) edges inside every block (none across); every
+    edge inserted in seeded random order, then `churn` of the edges deleted and reinserted, each
+    reinsert after its delete.  Returns (pairs, types, block_of_vertex)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = rng.integers(small[0], small[1] + 1, nblocks - 1)
+    sizes = np.concatenate([[n - int(sizes.sum())], sizes]).astype(np.int64)
+    perm = rng.permutation(n).astype(np.uint32)
+    block = np.empty(n, dtype=np.int32)
+    edges = []
+    start = 0
+    for b, s in enumerate(sizes):
+        ids = perm[start:start + s]
+        block[ids] = b
+        start += s
+        m = s * (s - 1) // 2
+        if m == 0:
+            continue
+        pos = []
+        cur = -1
+        while cur < m:
+            gaps = rng.geometric(p, size=int(max(64, (m - cur) * p * 1.1)))
+            run = cur + np.cumsum(gaps)
+            pos.append(run[run < m])
+            cur = int(run[-1])
+        k = np.concatenate(pos)
+        i, j = _tri_pairs(int(s), k)
+        edges.append(np.stack([ids[i], ids[j]], axis=1))
+    e = np.concatenate(edges).astype(np.uint32)
+    e = e[rng.permutation(e.shape[0])]
+    swap = rng.random(e.shape[0]) < 0.5
+    e[swap] = e[swap][:, ::-1]
+    m = e.shape[0]
+    nc = int(round(churn * m))
+    ch = rng.choice(m, size=nc, replace=False)
+    dkey = rng.random(nc)
+    rkey = dkey + (1 - dkey) * rng.random(nc)
+    keys = np.concatenate([dkey, rkey])
+    tail = np.concatenate([e[ch], e[ch]])
+    ttypes = np.concatenate([np.ones(nc, np.uint8), np.zeros(nc, np.uint8)])
+    order = np.argsort(keys, kind="stable")
+    pairs = np.concatenate([e, tail[order]])
+    types = np.concatenate([np.zeros(m, np.uint8), ttypes[order]])
+    return pairs, types, block
