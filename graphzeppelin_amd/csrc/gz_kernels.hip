@@ -190,8 +190,8 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
 // the LDS banks) instead of all hitting depth 0 of one column.  The deterministic bucket is kept
 // in registers per lane and reduced once per tile.  The write-back is a coalesced
 // read-xor-write of the vertex's bucket planes, or device-scope atomic XOR when the vertex is
-// split across tiles of the same launch.  The grid is persistent: workgroups pull tiles from an
-// atomic counter until the counter passes the tile count.
+// split across tiles of the same launch.  Workgroups walk the tiles grid-stride, so a persistent
+// grid can cover a tile count that only the device knows.
 // ================================================================================================
 template <class G, bool RAW, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
@@ -217,7 +217,9 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t start_col = lane % C;
   const uint64_t depth_cap = 1ull << D;
-  if (threadIdx.x == 0) s_tile = work_counter ? atomicAdd(work_counter, 1u) : blockIdx.x;
+  // Tiles are taken grid-stride (t, t + gridDim.x, ...): no shared counter, so no device-wide
+  // atomic on one hot address per tile.
+  if (threadIdx.x == 0) s_tile = blockIdx.x;
   __syncthreads();
 
   while (true) {
@@ -252,8 +254,8 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
       lds_xor(s_g + C * Dp, det_g);
     }
     __syncthreads();
-    // prefetch the next tile while this one is written back (read after the barrier below)
-    if (threadIdx.x == 0) s_tile = work_counter ? atomicAdd(work_counter, 1u) : ntiles;
+    // next tile, read after the barrier below
+    if (threadIdx.x == 0) s_tile = t + gridDim.x;
 
     // write-back: read-xor-write of the vertex rows, four buckets per thread in flight
     const size_t base = (size_t)tl.vertex * g.stride;
@@ -916,13 +918,10 @@ void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_d
                          uint32_t grid_cap, uint32_t* work_counter, const uint32_t* dsts, const uint64_t* raw_idx,
                          uint64_t* alpha, void* gamma, const Geo& g, const uint64_t* col_bf, hipStream_t s) {
   constexpr int B = 256;
-  uint32_t grid = grid_cap;
-  if (work_counter) {
-    if (!ntiles_dev) grid = ntiles_host < grid_cap ? ntiles_host : grid_cap;
-    launch_fill_u32(work_counter, 0, 1, s);
-  } else {
-    grid = ntiles_host;  // static mapping: one workgroup per tile
-  }
+  (void)work_counter;
+  // tile count known on the host: one workgroup per tile (capped, then grid-stride);
+  // known only on the device: a persistent grid of grid_cap workgroups
+  uint32_t grid = ntiles_dev ? grid_cap : (grid_cap && ntiles_host > grid_cap ? grid_cap : ntiles_host);
   if (grid == 0) return;
   const size_t lds = update_tiles_lds_bytes(g, gbytes);
   if (gbytes == 4) {

@@ -33,9 +33,9 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);
 
 // ---- sketch update ----
-// Persistent tile kernel.  `ntiles_dev` (device scalar) bounds the tile index when non-null,
-// otherwise `ntiles_host`.  With `work_counter` the grid is persistent (the launcher clears the
-// counter); with work_counter == nullptr workgroup b processes tile b (grid = ntiles_host).
+// Tile kernel.  `ntiles_dev` (device scalar) bounds the tile index when non-null (persistent grid
+// of grid_cap workgroups), otherwise `ntiles_host` (one workgroup per tile, at most grid_cap when
+// grid_cap > 0).  Workgroups walk tiles grid-stride.  `work_counter` is unused.
 // Exactly one of `dsts` (destination vertex ids; index = concat(vertex, dst)) or `raw_idx`
 // (64-bit indices) is non-null.
 void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
