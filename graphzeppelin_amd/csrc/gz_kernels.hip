@@ -83,19 +83,21 @@ __device__ inline uint32_t dsu_find(uint32_t* parent, uint32_t u) {
     p = ld_acq(parent + u);
   }
 }
-__device__ inline bool dsu_merge(uint32_t* parent, uint32_t a, uint32_t b, uint32_t pseed) {
-  // CAS union ordered by priority, dsu.h:155-167
+constexpr uint32_t kNoLink = 0xFFFFFFFFu;
+// CAS union ordered by priority (dsu.h:155-167).  Returns the root that was linked under the other
+// one, or kNoLink when a and b were already connected.
+__device__ inline uint32_t dsu_link(uint32_t* parent, uint32_t a, uint32_t b, uint32_t pseed) {
   while (true) {
     a = dsu_find(parent, a);
     b = dsu_find(parent, b);
-    if (a == b) return false;
+    if (a == b) return kNoLink;
     uint32_t pa = dsu_prio(a, pseed), pb = dsu_prio(b, pseed);
     if (pa == pb && a > b) { uint32_t t = a; a = b; b = t; }
     if (pa < pb) { uint32_t t = a; a = b; b = t; }
     uint32_t expected = b;
     if (__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT))
-      return true;
+      return b;
   }
 }
 
@@ -126,12 +128,14 @@ __device__ inline bool sf_contains(const DsuDev& d, uint64_t key) {
   }
   return false;
 }
-// A successful merge records the edge in the spanning forest (cc_sketch_alg.cpp:152-162)
-__device__ inline void sf_record(const DsuDev& d, uint32_t u, uint32_t v) {
-  uint32_t a = u < v ? u : v, b = u < v ? v : u;
-  sf_insert(d, ((uint64_t)a << 32) | b);
-  uint32_t k = __hip_atomic_fetch_add(d.flags + kFlagSfCount, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (k < d.n) { d.sf_list[2 * k] = a; d.sf_list[2 * k + 1] = b; }
+// A successful merge records the edge in the spanning forest (cc_sketch_alg.cpp:152-162): in the
+// slot of the root it linked (every root is linked at most once between resets, so slots never
+// collide) and in the membership set used by the eager DSU.
+__device__ inline void sf_record(const DsuDev& d, uint32_t child, uint32_t u, uint32_t v) {
+  const uint32_t a = u < v ? u : v, b = u < v ? v : u;
+  const uint64_t key = ((uint64_t)a << 32) | b;
+  d.sf_edge[child] = key;
+  sf_insert(d, key);
 }
 
 // ---- sample (src/sketch.cpp:94-116) over one wave --------------------------------------------
@@ -154,20 +158,20 @@ __device__ inline int wave_sample(uint64_t det_a, G det_g, uint64_t a, G gm, boo
 }
 
 // Act on a supernode's sample (sample_supernode, src/cc_sketch_alg.cpp:137-166).  Lane 0 only.
-__device__ inline void act_on_sample(const DsuDev& d, int code, uint64_t idx, uint32_t* modified) {
-  if (code == kFail) { st_rlx(modified, 1u); return; }
-  if (code != kGood) return;
-  uint32_t u = (uint32_t)(idx >> 32), v = (uint32_t)idx;
+// Returns true when the round is modified (FAIL, or a merge).
+__device__ inline bool act_on_sample(const DsuDev& d, int code, uint64_t idx) {
+  if (code == kFail) return true;
+  if (code != kGood) return false;
+  const uint32_t u = (uint32_t)(idx >> 32), v = (uint32_t)idx;
   if (u >= d.n || v >= d.n) {
     // index decoded from a checksum collision; counted as a failed sample
-    st_rlx(modified, 1u);
-    st_rlx(d.flags + kFlagBad, 1u);
-    return;
+    d.flags[kFlagBad] = 1;
+    return true;
   }
-  if (dsu_merge(d.parent, u, v, d.prio_seed)) {
-    sf_record(d, u, v);
-    st_rlx(modified, 1u);
-  }
+  const uint32_t child = dsu_link(d.parent, u, v, d.prio_seed);
+  if (child == kNoLink) return false;
+  sf_record(d, child, u, v);
+  return true;
 }
 
 // ================================================================================================
@@ -472,14 +476,16 @@ __global__ void k_eager_union(const uint2* __restrict__ pairs, size_t count, uin
       st_rlx(d.flags + kFlagBadInput, 1u);
       continue;
     }
-    merged[i] = dsu_merge(d.parent, p.x, p.y, d.prio_seed) ? 1 : 0;
+    const uint32_t child = dsu_link(d.parent, p.x, p.y, d.prio_seed);
+    merged[i] = child != kNoLink ? 1 : 0;
+    if (child != kNoLink) d.sf_edge[child] = concat_pairing(p.x, p.y);
   }
 }
 __global__ void k_eager_sf_insert(const uint2* __restrict__ pairs, size_t count, const uint8_t* __restrict__ merged,
                                   DsuDev d) {
   if (ld_acq(d.flags + kFlagEagerInvalid)) return;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
-    if (merged[i]) sf_record(d, pairs[i].x, pairs[i].y);
+    if (merged[i]) sf_insert(d, concat_pairing(pairs[i].x, pairs[i].y));
 }
 __global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, const uint8_t* __restrict__ merged,
                                  DsuDev d) {
@@ -488,18 +494,21 @@ __global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, 
     if (merged[i]) continue;
     const uint2 p = pairs[i];
     if (p.x == p.y || p.x >= d.n || p.y >= d.n) continue;
-    if (sf_contains(d, concat_pairing(p.x, p.y))) st_rlx(d.flags + kFlagEagerInvalid, 1u);
+    if (sf_contains(d, concat_pairing(p.x, p.y))) d.flags[kFlagEagerInvalid] = 1;  // same value from every writer
   }
 }
 
 // ================================================================================================
 // Borůvka (boruvka_emulation, src/cc_sketch_alg.cpp:464-513)
 // ================================================================================================
-__global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uint64_t sf_cap, uint32_t* flags,
-                            uint32_t* mod, uint32_t nmod) {
+__global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uint64_t sf_cap, uint64_t* sf_edge,
+                            uint32_t* flags, uint32_t* mod, uint32_t nmod) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (size_t i = t; i < n; i += stride) parent[i] = (uint32_t)i;
+  for (size_t i = t; i < n; i += stride) {
+    parent[i] = (uint32_t)i;
+    sf_edge[i] = kEmpty;
+  }
   for (size_t i = t; i < sf_cap; i += stride) sf_hash[i] = kEmpty;
   if (t < kNumFlags) flags[t] = 0;
   for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
@@ -512,6 +521,7 @@ __global__ void __launch_bounds__(256)
 k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, DsuDev d) {
   const uint32_t lane = lane_id();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
+  bool modified = false;
   for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < g.n; v += waves) {
     const size_t base = (size_t)v * g.stride;
     const uint64_t det_a = alpha[base + g.nb - 1];
@@ -521,81 +531,131 @@ k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma,
     const G gm = valid ? gamma[base + lane] : 0;
     uint64_t idx;
     const int code = wave_sample<G>(det_a, det_g, a, gm, valid, g.chk_bf, &idx);
-    if (lane == 0) act_on_sample(d, code, idx, d.mod);
+    if (lane == 0 && act_on_sample(d, code, idx)) modified = true;
   }
+  // plain store: every writer stores the same value, so no device-wide atomic is needed
+  if (lane == 0 && modified) d.mod[0] = 1;
 }
 
 // ---- rounds >= 1 (perform_boruvka_round 246-380 + create_merge_instructions 382-462) ----------
 // Every kernel of round r first reads mod[r-1]: when round r-1 changed nothing the whole round is
 // a no-op, which lets the host queue several rounds before it looks at the flags.
 //
-// Regrouping by root: A) root of every vertex + wave-aggregated count per root; B) each root gets
-// a contiguous segment [off, off+cnt) from one running cursor (segments are contiguous but not in
-// root order, which Borůvka does not need); C) scatter the vertices into their segments.
-__global__ void k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root,
-                                uint32_t* __restrict__ cnt, uint32_t* __restrict__ seg_cursor,
-                                const uint32_t* __restrict__ prev_mod) {
+// Regrouping by root: A) root of every vertex and members per root; B) each root gets a contiguous
+// segment [off, off+cnt) from one running cursor (contiguous, not in root order, which Borůvka does
+// not need); C) scatter the vertices into their segments.  Counts and cursors are aggregated per
+// 1024-thread block in an LDS table first, so a root shared by the whole block (the giant component
+// of later rounds) costs one device atomic per block instead of one per wave or per vertex.
+constexpr int kAggBlock = 1024;
+constexpr uint32_t kAggTable = 2048;  // >= 2 * kAggBlock, power of two
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+
+__device__ inline uint32_t lds_table_slot(uint32_t* keys, uint32_t key) {
+  uint32_t h = (key * 2654435761u) & (kAggTable - 1);
+  while (true) {
+    const uint32_t prev = atomicCAS(keys + h, kEmptyKey, key);
+    if (prev == kEmptyKey || prev == key) return h;
+    h = (h + 1) & (kAggTable - 1);
+  }
+}
+
+__global__ void __launch_bounds__(kAggBlock)
+k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root, uint32_t* __restrict__ cnt,
+                uint32_t* __restrict__ seg_cursor, const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
+  __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable];
+  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock) {
+    t_key[i] = kEmptyKey;
+    t_cnt[i] = 0;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) *seg_cursor = 0;
+  __syncthreads();
   const uint32_t lane = lane_id();
-  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
-    const uint32_t v = v0 + lane;
-    const bool active = v < n;
-    uint32_t r = 0;
-    if (active) {
-      r = dsu_find(parent, v);
-      root[v] = r;
-    }
-    uint64_t todo = __ballot(active);
-    while (todo) {
-      const int leader = __builtin_ctzll(todo);
-      const uint32_t lr = rd_lane_u32(r, leader);
-      const uint64_t same = __ballot(active && r == lr) & todo;
-      if ((int)lane == leader) atomicAdd(cnt + lr, (uint32_t)__popcll(same));
-      todo &= ~same;
-    }
+  const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool active = v < n;
+  uint32_t r = 0;
+  if (active) {
+    r = dsu_find(parent, v);
+    root[v] = r;
+  }
+  uint64_t todo = __ballot(active);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const uint32_t lr = rd_lane_u32(r, leader);
+    const uint64_t same = __ballot(active && r == lr) & todo;
+    if ((int)lane == leader) atomicAdd(t_cnt + lds_table_slot(t_key, lr), (uint32_t)__popcll(same));
+    todo &= ~same;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock)
+    if (t_key[i] != kEmptyKey) atomicAdd(cnt + t_key[i], t_cnt[i]);
+}
+
+__global__ void __launch_bounds__(kAggBlock)
+k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+           uint32_t* __restrict__ seg_cursor, const uint32_t* __restrict__ prev_mod) {
+  if (!ld_acq(prev_mod)) return;
+  __shared__ uint64_t wsum[kAggBlock / kWave];
+  __shared__ uint32_t s_base;
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
+  const uint32_t c = v < n ? cnt[v] : 0;
+  const uint64_t incl = wave_incl_scan_u64(c);
+  if (lane == kWave - 1) wsum[wid] = incl;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+  for (uint32_t w = 0; w < kAggBlock / kWave; ++w) {
+    if (w < wid) pre += wsum[w];
+    tot += wsum[w];
+  }
+  if (threadIdx.x == 0) s_base = tot ? atomicAdd(seg_cursor, (uint32_t)tot) : 0;
+  __syncthreads();
+  if (c) {
+    const uint32_t o = s_base + (uint32_t)(pre + incl - c);
+    off[v] = o;
+    cursor[v] = o;
   }
 }
-__global__ void k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off,
-                           uint32_t* __restrict__ cursor, uint32_t* __restrict__ seg_cursor,
-                           const uint32_t* __restrict__ prev_mod) {
+
+__global__ void __launch_bounds__(kAggBlock)
+k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
+             uint32_t* __restrict__ members, const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
-  const uint32_t lane = lane_id();
-  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
-    const uint32_t v = v0 + lane;
-    const uint32_t c = v < n ? cnt[v] : 0;
-    const uint64_t incl = wave_incl_scan_u64(c);
-    uint32_t base = 0;
-    if (lane == kWave - 1 && incl) base = atomicAdd(seg_cursor, (uint32_t)incl);
-    base = rd_lane_u32(base, kWave - 1);
-    if (c) {
-      const uint32_t o = base + (uint32_t)(incl - c);
-      off[v] = o;
-      cursor[v] = o;
-    }
+  __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable], t_base[kAggTable];
+  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock) {
+    t_key[i] = kEmptyKey;
+    t_cnt[i] = 0;
   }
-}
-__global__ void k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
-                             uint32_t* __restrict__ members, const uint32_t* __restrict__ prev_mod) {
-  if (!ld_acq(prev_mod)) return;
+  __syncthreads();
   const uint32_t lane = lane_id();
   const uint64_t lt = (1ull << lane) - 1;
-  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane; v0 < n; v0 += gridDim.x * blockDim.x) {
-    const uint32_t v = v0 + lane;
-    const bool active = v < n;
-    const uint32_t r = active ? root[v] : 0;
-    uint64_t todo = __ballot(active);
-    while (todo) {
-      const int leader = __builtin_ctzll(todo);
-      const uint32_t lr = rd_lane_u32(r, leader);
-      const uint64_t same = __ballot(active && r == lr) & todo;
-      uint32_t b = 0;
-      if ((int)lane == leader) b = atomicAdd(cursor + lr, (uint32_t)__popcll(same));
-      b = __shfl(b, leader);
-      if ((same >> lane) & 1) members[b + __popcll(same & lt)] = v;
-      todo &= ~same;
+  const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool active = v < n;
+  const uint32_t r = active ? root[v] : 0;
+  uint32_t my_slot = 0, my_pos = 0;
+  uint64_t todo = __ballot(active);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const uint32_t lr = rd_lane_u32(r, leader);
+    const uint64_t same = __ballot(active && r == lr) & todo;
+    uint32_t slot = 0, at = 0;
+    if ((int)lane == leader) {
+      slot = lds_table_slot(t_key, lr);
+      at = atomicAdd(t_cnt + slot, (uint32_t)__popcll(same));  // position of this group in the block's share
     }
+    slot = __shfl(slot, leader);
+    at = __shfl(at, leader);
+    if ((same >> lane) & 1) {
+      my_slot = slot;
+      my_pos = at + (uint32_t)__popcll(same & lt);
+    }
+    todo &= ~same;
   }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock)
+    if (t_key[i] != kEmptyKey) t_base[i] = atomicAdd(cursor + t_key[i], t_cnt[i]);
+  __syncthreads();
+  if (active) members[t_base[my_slot] + my_pos] = v;
 }
 
 // Segmented XOR-reduce + sample, one level (range_merge of column `round` + the deterministic
@@ -645,6 +705,7 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
   const bool loads = lane <= g.D;
   uint64_t* out_a = out_part + (size_t)(2 * wave) * 2 * kWave;  // entry layout: alpha[64] then gamma[64]
   bool wrote_head = false, wrote_tail = false;
+  bool wave_mod = false;
 
   auto emit = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g, uint32_t seg_start, uint32_t seg_end) {
     const bool started_before = seg_start == p0 && head_open;
@@ -655,7 +716,7 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
       const G det_g = (G)rd_lane_u64(acc_g, g.D);
       uint64_t idx;
       const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
-      if (lane == 0) act_on_sample(d, code, idx, cur_mod);
+      if (lane == 0 && act_on_sample(d, code, idx)) wave_mod = true;
       return;
     }
     // Partial of a spanning supernode: the head slot if it started before this wave, else the tail
@@ -679,7 +740,7 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
 
   uint64_t acc_a = 0, acc_g = 0;
   uint32_t cur = first_root, seg_start = p0;
-  constexpr uint32_t U = 4;
+  constexpr uint32_t U = FIRST ? 8 : 16;  // loads in flight per lane
   for (uint32_t j = 0; j < len; j += U) {
     uint64_t va[U], vg[U];
 #pragma unroll
@@ -715,6 +776,8 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
     }
   }
   emit(cur, acc_a, acc_g, seg_start, p1);
+  // plain store: every writer stores the same value
+  if (lane == 0 && wave_mod) *cur_mod = 1;
   // unused output slots carry no root and no value
   if (!wrote_head) {
     out_a[lane] = 0;
@@ -964,7 +1027,7 @@ void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const Ds
 void launch_dsu_reset(const DsuDev& d, hipStream_t s) {
   const size_t work = std::max<size_t>(d.n, d.sf_mask + 1);
   hipLaunchKernelGGL(k_dsu_reset, dim3(grid_for(work, 256)), dim3(256), 0, s, d.parent, d.n, d.sf_hash,
-                     d.sf_mask + 1, d.flags, d.mod, d.nmod);
+                     d.sf_mask + 1, d.sf_edge, d.flags, d.mod, d.nmod);
 }
 
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
@@ -995,10 +1058,10 @@ void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void*
   const uint32_t* prev = d.mod + (round - 1);
   uint32_t* cur = d.mod + round;
   uint32_t* cnt = rb.cnt;  // zero on entry; the level-1 reduce clears it again for the next round
-  const uint32_t grid = grid_for(n, 256);
-  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(256), 0, s, d.parent, n, rb.root, cnt, rb.totals, prev);
-  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(256), 0, s, cnt, n, rb.off, rb.cursor, rb.totals, prev);
-  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(256), 0, s, rb.root, n, rb.cursor, rb.members, prev);
+  const uint32_t grid = (n + kAggBlock - 1) / kAggBlock;  // one vertex per thread
+  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, cnt, rb.totals, prev);
+  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, cnt, n, rb.off, rb.cursor, rb.totals, prev);
+  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, rb.members, prev);
   uint32_t ent[32];
   const size_t L = round_levels(n, ent, 32);
   for (size_t l = 0; l < L; ++l) {

@@ -11,7 +11,7 @@ struct DsuDev {
   uint32_t* parent;    // n
   uint64_t* sf_hash;   // open-addressing set of spanning-forest edge ids, cap = sf_mask + 1
   uint64_t sf_mask;
-  uint32_t* sf_list;   // 2 * (n - 1) endpoints, appended in merge order
+  uint64_t* sf_edge;   // n: forest edge (min << 32 | max) that linked root v under another, or ~0
   uint32_t* flags;     // FlagSlot entries
   uint32_t* mod;       // per-round "modified" flags, mod[r] for round r (nmod entries)
   uint32_t nmod;
@@ -22,7 +22,7 @@ struct DsuDev {
 enum FlagSlot {
   kFlagModified = 0,
   kFlagEagerInvalid = 1,
-  kFlagSfCount = 2,
+  kFlagUnused = 2,
   kFlagBad = 3,       // a sample decoded to an out-of-range vertex
   kFlagBadInput = 4,  // an ingested pair named a vertex >= n (the pair was skipped)
   kNumFlags = 8

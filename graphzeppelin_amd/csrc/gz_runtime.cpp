@@ -206,14 +206,14 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   uint64_t cap = 1;
   while (cap < 2ull * n) cap <<= 1;
   sf_hash.alloc(cap);
-  sf_list.alloc(2ull * n);
+  sf_edge.alloc(n);
   flags.alloc(kNumFlags);
   mod.alloc(geo.samples + 2);
   labels.alloc(n + 1);
   dsu.parent = parent.p;
   dsu.sf_hash = sf_hash.p;
   dsu.sf_mask = cap - 1;
-  dsu.sf_list = sf_list.p;
+  dsu.sf_edge = sf_edge.p;
   dsu.flags = flags.p;
   dsu.mod = mod.p;
   dsu.nmod = geo.samples + 2;
@@ -654,14 +654,21 @@ void GzAlg::point_queries(const uint32_t* us, const uint32_t* vs, size_t count, 
   GZ_HIP(hipStreamSynchronize(st));
 }
 
-// calc_spanning_forest, src/cc_sketch_alg.cpp:559-568
+// calc_spanning_forest, src/cc_sketch_alg.cpp:559-568: every root linked under another records the
+// forest edge that linked it; the forest is the list of those edges.
 size_t GzAlg::spanning_forest(uint32_t* edges, size_t cap) {
   connected_components(nullptr);
-  const uint32_t k = std::min<uint32_t>(read_flag(kFlagSfCount), n);
-  if (edges && cap) {
-    const size_t c = std::min<size_t>(cap, k);
-    GZ_HIP(hipMemcpyAsync(edges, sf_list.p, c * 8, hipMemcpyDeviceToHost, st));
-    GZ_HIP(hipStreamSynchronize(st));
+  std::vector<uint64_t> slots(n);
+  GZ_HIP(hipMemcpyAsync(slots.data(), sf_edge.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+  size_t k = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    if (slots[v] == ~0ull) continue;
+    if (edges && k < cap) {
+      edges[2 * k] = (uint32_t)(slots[v] >> 32);
+      edges[2 * k + 1] = (uint32_t)slots[v];
+    }
+    ++k;
   }
   return k;
 }

@@ -135,7 +135,8 @@ class GzAlg {
   DevBuf<uint64_t> alpha, col_bf;
   DevBuf<uint8_t> gamma;
   // DSU / forest
-  DevBuf<uint32_t> parent, sf_list, flags, labels, qa, qb;
+  DevBuf<uint32_t> parent, flags, labels, qa, qb;
+  DevBuf<uint64_t> sf_edge;
   DevBuf<uint64_t> sf_hash;
   DevBuf<uint8_t> qout;
   DsuDev dsu{};
