@@ -30,6 +30,10 @@ __device__ inline uint64_t shfl_u64(uint64_t v, int src) {
   uint32_t lo = __shfl((unsigned)(uint32_t)v, src), hi = __shfl((unsigned)(uint32_t)(v >> 32), src);
   return ((uint64_t)hi << 32) | lo;
 }
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int o) {
+  uint32_t lo = __shfl_xor((unsigned)(uint32_t)v, o), hi = __shfl_xor((unsigned)(uint32_t)(v >> 32), o);
+  return ((uint64_t)hi << 32) | lo;
+}
 __device__ inline uint64_t wave_xor_u64(uint64_t v) {
   for (int o = 32; o > 0; o >>= 1) {
     uint32_t lo = __shfl_xor((unsigned)(uint32_t)v, o), hi = __shfl_xor((unsigned)(uint32_t)(v >> 32), o);
@@ -83,6 +87,18 @@ __device__ inline uint32_t dsu_find(uint32_t* parent, uint32_t u) {
     p = ld_acq(parent + u);
   }
 }
+// find for kernels in which no thread links roots (only compression writes, which keep every
+// pointer on the path to the root): cached loads are enough.
+__device__ inline uint32_t dsu_find_stable(uint32_t* parent, uint32_t u) {
+  uint32_t p = parent[u];
+  while (true) {
+    const uint32_t gp = parent[p];
+    if (gp == p) return p;
+    u = p;
+    p = gp;
+  }
+}
+
 constexpr uint32_t kNoLink = 0xFFFFFFFFu;
 // CAS union ordered by priority (dsu.h:155-167).  Returns the root that was linked under the other
 // one, or kNoLink when a and b were already connected.
@@ -138,6 +154,24 @@ __device__ inline void sf_record(const DsuDev& d, uint32_t child, uint32_t u, ui
   sf_insert(d, key);
 }
 
+// Sample records of round 0: a GOOD sample stores its edge index, whose endpoints are both below
+// n <= 2^32 - 1 after the range check, so it never equals the two sentinels (high half all ones)
+// that stand for ZERO and FAIL.
+constexpr uint64_t kSampZero = ~0ull;
+constexpr uint64_t kSampFail = ~0ull - 1;
+
+// An index whose endpoints are out of range (a checksum collision) counts as FAIL and raises the
+// bad-sample flag (plain store: every writer stores the same value).
+__device__ inline uint64_t sample_record(int code, uint64_t idx, uint32_t n, uint32_t* flags) {
+  if (code == kZero) return kSampZero;
+  if (code != kGood) return kSampFail;
+  if ((uint32_t)(idx >> 32) >= n || (uint32_t)idx >= n) {
+    flags[kFlagBad] = 1;
+    return kSampFail;
+  }
+  return idx;
+}
+
 // ---- sample (src/sketch.cpp:94-116) over one wave --------------------------------------------
 // Lane l < D holds bucket (column, depth l); every lane holds the deterministic bucket.
 // Returns the code; *idx gets the sampled index on GOOD.
@@ -155,23 +189,6 @@ __device__ inline int wave_sample(uint64_t det_a, G det_g, uint64_t a, G gm, boo
   }
   *idx = 0;
   return kFail;
-}
-
-// Act on a supernode's sample (sample_supernode, src/cc_sketch_alg.cpp:137-166).  Lane 0 only.
-// Returns true when the round is modified (FAIL, or a merge).
-__device__ inline bool act_on_sample(const DsuDev& d, int code, uint64_t idx) {
-  if (code == kFail) return true;
-  if (code != kGood) return false;
-  const uint32_t u = (uint32_t)(idx >> 32), v = (uint32_t)idx;
-  if (u >= d.n || v >= d.n) {
-    // index decoded from a checksum collision; counted as a failed sample
-    d.flags[kFlagBad] = 1;
-    return true;
-  }
-  const uint32_t child = dsu_link(d.parent, u, v, d.prio_seed);
-  if (child == kNoLink) return false;
-  sf_record(d, child, u, v);
-  return true;
 }
 
 // ================================================================================================
@@ -430,16 +447,34 @@ k_scan_apply(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t tile, const 
     run += pack_ct(c[k], tile);
   }
 }
-__global__ void k_make_tiles(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
-                             const uint32_t* __restrict__ toff, uint32_t n, uint32_t tile, Tile* __restrict__ tiles) {
+// Also accumulates, when acc is given, the flush's algorithmic bytes (sum over vertices of
+// 4k + 2*min(S, k*(C+1)*B)) into acc[blockIdx.x]: one slot per block, so no atomics; the grid
+// never exceeds kProfSlots blocks and launches on one stream are ordered.
+__global__ void __launch_bounds__(256)
+k_make_tiles(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, const uint32_t* __restrict__ toff,
+             uint32_t n, uint32_t tile, Tile* __restrict__ tiles, uint64_t* __restrict__ acc, uint64_t S,
+             uint64_t per_update) {
+  uint64_t bytes = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     const uint32_t c = cnt[v];
     if (!c) continue;
+    const uint64_t touched = (uint64_t)c * per_update;
+    bytes += 4ull * c + 2 * (touched < S ? touched : S);
     const uint32_t nt = (c + tile - 1) / tile, o = off[v], t0 = toff[v];
     for (uint32_t j = 0; j < nt; ++j) {
       const uint32_t b = j * tile;
       tiles[t0 + j] = Tile{v, o + b, (c - b) < tile ? (c - b) : tile, nt > 1 ? 1u : 0u};
     }
+  }
+  if (!acc) return;  // uniform over the grid
+  __shared__ uint64_t s_part[256 / kWave];
+  for (int o = 32; o > 0; o >>= 1) bytes += shfl_xor_u64(bytes, o);
+  if (lane_id() == 0) s_part[threadIdx.x / kWave] = bytes;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < 256 / kWave; ++w) t += s_part[w];
+    acc[blockIdx.x] += t;
   }
 }
 
@@ -518,10 +553,10 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
 // per vertex; lane l checks depth l, the first good depth is the lowest set bit of the ballot.
 template <class G>
 __global__ void __launch_bounds__(256)
-k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, DsuDev d) {
+k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint64_t* __restrict__ samp,
+                uint32_t* __restrict__ flags) {
   const uint32_t lane = lane_id();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
-  bool modified = false;
   for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < g.n; v += waves) {
     const size_t base = (size_t)v * g.stride;
     const uint64_t det_a = alpha[base + g.nb - 1];
@@ -531,10 +566,33 @@ k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma,
     const G gm = valid ? gamma[base + lane] : 0;
     uint64_t idx;
     const int code = wave_sample<G>(det_a, det_g, a, gm, valid, g.chk_bf, &idx);
-    if (lane == 0 && act_on_sample(d, code, idx)) modified = true;
+    if (lane == 0) samp[v] = sample_record(code, idx, g.n, flags);
   }
-  // plain store: every writer stores the same value, so no device-wide atomic is needed
-  if (lane == 0 && modified) d.mod[0] = 1;
+}
+
+// The merges of a round (sample_supernode, src/cc_sketch_alg.cpp:137-166), one thread per
+// supernode: every vertex that was a root when the round began (root == nullptr: every vertex, as
+// in round 0) acts on its sample record.  The round is modified by a FAIL or a merge.
+__global__ void __launch_bounds__(256)
+k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ root, DsuDev d,
+                const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod) {
+  if (prev_mod && !ld_acq(prev_mod)) return;
+  bool modified = false;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += gridDim.x * blockDim.x) {
+    if (root && root[v] != v) continue;
+    const uint64_t s = samp[v];
+    if (s == kSampZero) continue;
+    if (s == kSampFail) {
+      modified = true;
+      continue;
+    }
+    const uint32_t a = (uint32_t)(s >> 32), b = (uint32_t)s;
+    const uint32_t child = dsu_link(d.parent, a, b, d.prio_seed);
+    if (child == kNoLink) continue;
+    sf_record(d, child, a, b);
+    modified = true;
+  }
+  if (__syncthreads_or(modified) && threadIdx.x == 0) *cur_mod = 1;  // one plain store per block
 }
 
 // ---- rounds >= 1 (perform_boruvka_round 246-380 + create_merge_instructions 382-462) ----------
@@ -575,9 +633,12 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
   const bool active = v < n;
   uint32_t r = 0;
   if (active) {
-    r = dsu_find(parent, v);
+    r = dsu_find_stable(parent, v);
     root[v] = r;
   }
+  __syncthreads();
+  // full path compression: every vertex now points at its root (no links happen in this kernel)
+  if (active) parent[v] = r;
   uint64_t todo = __ballot(active);
   while (todo) {
     const int leader = __builtin_ctzll(todo);
@@ -662,7 +723,8 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
 // bucket over every member of a supernode, then sample_supernode).  Entries are grouped by root;
 // a wave owns 64 consecutive entries, lane l < D accumulates depth l of the column and lane D the
 // deterministic bucket.  A supernode whose entries all lie inside the wave is sampled from
-// registers.  A supernode that crosses the wave's first (head) or last (tail) boundary leaves its
+// registers; its sample record goes to samp[root] for k_apply_samples.  A supernode that crosses
+// the wave's first (head) or last (tail) boundary leaves its
 // partial XOR in out[2w] / out[2w+1] under its root; unused outputs get root kNoRoot and zeros.
 // The next level reduces those 2*waves entries the same way, so a supernode spanning many waves is
 // combined without any atomics; the last level has a single wave.
@@ -673,8 +735,8 @@ __global__ void __launch_bounds__(256)
 k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
             const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint32_t round,
             const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
-            uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, DsuDev d,
-            const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod, uint32_t* __restrict__ zero_cnt) {
+            uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
+            uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ zero_cnt) {
   if (!ld_acq(prev_mod)) return;
   const uint32_t lane = lane_id();
   if (FIRST && zero_cnt) {  // clear the count buffer the next round will use
@@ -705,7 +767,6 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
   const bool loads = lane <= g.D;
   uint64_t* out_a = out_part + (size_t)(2 * wave) * 2 * kWave;  // entry layout: alpha[64] then gamma[64]
   bool wrote_head = false, wrote_tail = false;
-  bool wave_mod = false;
 
   auto emit = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g, uint32_t seg_start, uint32_t seg_end) {
     const bool started_before = seg_start == p0 && head_open;
@@ -716,7 +777,7 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
       const G det_g = (G)rd_lane_u64(acc_g, g.D);
       uint64_t idx;
       const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
-      if (lane == 0 && act_on_sample(d, code, idx)) wave_mod = true;
+      if (lane == 0) samp[R] = sample_record(code, idx, g.n, flags);
       return;
     }
     // Partial of a spanning supernode: the head slot if it started before this wave, else the tail
@@ -776,8 +837,6 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
     }
   }
   emit(cur, acc_a, acc_g, seg_start, p1);
-  // plain store: every writer stores the same value
-  if (lane == 0 && wave_mod) *cur_mod = 1;
   // unused output slots carry no root and no value
   if (!wrote_head) {
     out_a[lane] = 0;
@@ -940,24 +999,6 @@ k_digest(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g,
   if (lane_id() == 0) atomicAdd(out, (unsigned long long)acc);
 }
 
-// Algorithmic bytes of a grouped flush: sum over vertices of 4k + 2*min(S, k*(C+1)*B).
-__global__ void k_flush_alg_bytes(const uint32_t* __restrict__ cnt, uint32_t n, uint64_t S, uint64_t per_update,
-                                  unsigned long long* __restrict__ out) {
-  uint64_t acc = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const uint64_t k = cnt[v];
-    if (k) {
-      const uint64_t touched = k * per_update;
-      acc += 4 * k + 2 * (touched < S ? touched : S);
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    uint32_t lo = __shfl_xor((unsigned)(uint32_t)acc, o), hi = __shfl_xor((unsigned)(uint32_t)(acc >> 32), o);
-    acc += ((uint64_t)hi << 32) | lo;
-  }
-  if (lane_id() == 0 && acc) atomicAdd(out, (unsigned long long)acc);
-}
-
 }  // namespace
 
 // ================================================================================================
@@ -1006,13 +1047,15 @@ void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_d
 
 size_t group_partials_len(uint32_t n) { return (n + kScanSpan - 1) / kScanSpan + 1; }
 
-void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile, const GroupBufs& b, hipStream_t s) {
+void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile, const GroupBufs& b, hipStream_t s,
+                  const FlushBytes* fb) {
   launch_fill_u32(b.cnt, 0, n, s);
   const uint2* p2 = reinterpret_cast<const uint2*>(pairs);
   if (count) hipLaunchKernelGGL(k_count_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cnt, b.bad);
   scan_counts(b.cnt, n, tile, b.partials, b.off, b.toff, b.cursor, b.totals, s);
   if (count) hipLaunchKernelGGL(k_scatter_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cursor, b.dst_arr);
-  hipLaunchKernelGGL(k_make_tiles, dim3(grid_for(n, 256)), dim3(256), 0, s, b.cnt, b.off, b.toff, n, tile, b.tiles);
+  hipLaunchKernelGGL(k_make_tiles, dim3(grid_for(n, 256, kProfSlots)), dim3(256), 0, s, b.cnt, b.off, b.toff, n, tile,
+                     b.tiles, fb ? fb->acc : nullptr, fb ? fb->S : 0, fb ? fb->per_update : 0);
 }
 
 void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d, hipStream_t s) {
@@ -1031,12 +1074,14 @@ void launch_dsu_reset(const DsuDev& d, hipStream_t s) {
 }
 
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
-                          hipStream_t s) {
+                          uint64_t* samp, hipStream_t s) {
   const uint32_t grid = grid_for((size_t)g.n * kWave, 256, 16384);
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, d);
+    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, samp, d.flags);
   else
-    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, d);
+    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, samp, d.flags);
+  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr,
+                     d.mod);
 }
 
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) {
@@ -1075,19 +1120,20 @@ void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void*
     if (l == 0) {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, cnt);
+                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, cnt);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, cnt);
+                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, cnt);
     } else {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, nullptr);
+                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, nullptr);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, d, prev, cur, nullptr);
+                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, nullptr);
     }
   }
+  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, prev, cur);
 }
 
 void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s) {
@@ -1098,12 +1144,6 @@ void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const G
     hipLaunchKernelGGL(k_digest<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, o);
   else
     hipLaunchKernelGGL(k_digest<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, o);
-}
-
-void launch_flush_alg_bytes(const uint32_t* cnt, uint32_t n, uint64_t S, uint64_t per_update, uint64_t* out,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_flush_alg_bytes, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, cnt, n, S, per_update,
-                     reinterpret_cast<unsigned long long*>(out));
 }
 
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s) {

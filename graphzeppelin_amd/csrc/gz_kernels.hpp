@@ -57,15 +57,25 @@ struct GroupBufs {
 };
 size_t group_partials_len(uint32_t n);
 // pairs: 2*count u32 (src,dst) in HBM.  Produces tiles of at most `tile` destinations.
-void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile, const GroupBufs& b, hipStream_t s);
+// Profiling: per-block accumulators of a grouped flush's algorithmic bytes (kProfSlots u64,
+// summed on the host).
+constexpr uint32_t kProfSlots = 8192;
+struct FlushBytes {
+  uint64_t* acc;
+  uint64_t S;           // bytes of one vertex sketch
+  uint64_t per_update;  // (C + 1) * (8 + sizeof(gamma)) bytes touched by one directed update
+};
+void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile, const GroupBufs& b, hipStream_t s,
+                  const FlushBytes* fb = nullptr);
 
 // ---- eager DSU (batch form of pre_insert, src/cc_sketch_alg.cpp:79-104) ----
 void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d, hipStream_t s);
 
 // ---- Borůvka ----
 void launch_dsu_reset(const DsuDev& d, hipStream_t s);
+// samp: n u64 sample records (the same buffer as RoundBufs::samp)
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
-                          hipStream_t s);
+                          uint64_t* samp, hipStream_t s);
 struct RoundBufs {
   uint32_t* root;      // n      root of every vertex this round
   uint32_t* cnt;       // n      members per root (zero between rounds)
@@ -75,6 +85,7 @@ struct RoundBufs {
   uint32_t* totals;    // [0]    segment allocation cursor
   uint64_t* part[2];   // partial-XOR entries of the multi-level reduce, 2*ceil(n/64) entries of 128 u64
   uint32_t* proot[2];  // root of each partial entry
+  uint64_t* samp;      // n      sample record per supernode root
 };
 // Number of reduce levels for n entries (entries_per_level gets each level's entry count).
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
@@ -89,8 +100,6 @@ void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* q
 void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s);
 
 // adds sum_v 4k + 2*min(S, k*per_update) over per-vertex counts k = cnt[v] to *out
-void launch_flush_alg_bytes(const uint32_t* cnt, uint32_t n, uint64_t S, uint64_t per_update, uint64_t* out,
-                            hipStream_t s);
 
 // ---- standalone sketch ops ----
 // out[0] = code, out[1..2] = idx (lo, hi)

@@ -223,14 +223,14 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
 
   // Borůvka round buffers
   const uint32_t waves1 = (n + 63) / 64;
-  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_members.alloc(n);
+  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_members.alloc(n); r_samp.alloc(n);
   r_totals.alloc(4);
   for (int i = 0; i < 2; ++i) {
     r_part[i].alloc((size_t)(2 * waves1 + 2) * 128);
     r_proot[i].alloc(2 * waves1 + 2);
   }
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, r_members.p, r_totals.p, {r_part[0].p, r_part[1].p},
-                 {r_proot[0].p, r_proot[1].p}};
+                 {r_proot[0].p, r_proot[1].p}, r_samp.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
@@ -466,21 +466,20 @@ void GzAlg::prof_collect() {
     ev_pool.push_back(t.b);
   }
   timed.clear();
+  std::vector<uint64_t> slots(kProfSlots);
+  GZ_HIP(hipMemcpy(slots.data(), prof_bytes.p, kProfSlots * 8, hipMemcpyDeviceToHost));
   uint64_t db = 0;
-  GZ_HIP(hipMemcpy(&db, prof_bytes.p, 8, hipMemcpyDeviceToHost));
+  for (uint64_t x : slots) db += x;
   profile.update.alg_bytes += (double)db;
-  launch_fill_u64(prof_bytes.p, 0, 1, st);
+  launch_fill_u64(prof_bytes.p, 0, kProfSlots, st);
   GZ_HIP(hipStreamSynchronize(st));
 }
 
 void GzAlg::flush_pending() {
   if (!pending_n) return;
-  launch_group(pending.p, pending_n, n, tile, gbufs, st);
-  if (prof) {
-    const uint64_t per_update = (uint64_t)(geo.C + 1) * (8 + gb);
-    launch_flush_alg_bytes(g_cnt.p, n, sketch_bytes(), per_update, prof_bytes.p, st);
-    prof_begin(0, 0, true, 2 * pending_n);
-  }
+  const FlushBytes fb{prof_bytes.p, sketch_bytes(), (uint64_t)(geo.C + 1) * (8 + gb)};
+  launch_group(pending.p, pending_n, n, tile, gbufs, st, prof ? &fb : nullptr);
+  if (prof) prof_begin(0, 0, true, 2 * pending_n);
   launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, work_counter.p, dst_arr.p, nullptr, alpha.p, gamma.p,
                       geo, col_bf.p, st);
   if (prof) prof_end();
@@ -585,7 +584,7 @@ void GzAlg::boruvka() {
     launch_dsu_reset(dsu, st);
     launch_fill_u32(r_cnt.p, 0, n, st);
     if (prof) prof_begin(1, 0, false, 0);
-    launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, st);
+    launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, r_samp.p, st);
     GZ_HIP(hipGetLastError());
     const uint32_t last_round = geo.samples - 1;  // round index `samples` would be out of samples
     uint32_t next = 1, group = 2;
@@ -652,6 +651,48 @@ void GzAlg::point_queries(const uint32_t* us, const uint32_t* vs, size_t count, 
   launch_point_queries(dsu, qa.p, qb.p, (uint32_t)count, qout.p, st);
   GZ_HIP(hipMemcpyAsync(outp, qout.p, count, hipMemcpyDeviceToHost, st));
   GZ_HIP(hipStreamSynchronize(st));
+}
+
+// One consistent state for a group of queries (the serving loop's unit of work): a single read of
+// the eager-DSU flag decides between the cached DSU and flush + Borůvka; then every connected-
+// components count and point query of the group is answered by one kernel pass and one copy back.
+bool GzAlg::answer_queries(const uint32_t* us, const uint32_t* vs, size_t npoint, bool want_cc, uint32_t* ncc_out,
+                           uint8_t* point_out) {
+  for (size_t i = 0; i < npoint; ++i)
+    if (us[i] >= n || vs[i] >= n) throw GzError(GZ_ERR_INVALID_ARG, "vertex id out of range");
+  std::lock_guard<std::mutex> lk(query_mtx);
+  sync_eager_state();
+  const bool cached = dsu_valid;
+  if (!cached) {
+    {
+      std::lock_guard<std::mutex> g(stage_mtx);
+      if (cfg.batched) submit_stage_locked();
+      flush_pending();
+    }
+    boruvka();
+  }
+  if (npoint) {
+    qa.ensure(npoint);
+    qb.ensure(npoint);
+    qout.ensure(npoint);
+    GZ_HIP(hipMemcpyAsync(qa.p, us, npoint * 4, hipMemcpyHostToDevice, st));
+    GZ_HIP(hipMemcpyAsync(qb.p, vs, npoint * 4, hipMemcpyHostToDevice, st));
+    launch_point_queries(dsu, qa.p, qb.p, (uint32_t)npoint, qout.p, st);
+    GZ_HIP(hipMemcpyAsync(point_out, qout.p, npoint, hipMemcpyDeviceToHost, st));
+  }
+  if (want_cc) {
+    launch_labels(dsu, labels.p, labels.p + n, st);
+    GZ_HIP(hipMemcpyAsync(hsmall.p, labels.p + n, 4, hipMemcpyDeviceToHost, st));
+  }
+  GZ_HIP(hipMemcpyAsync(hflags.p, flags.p, kNumFlags * 4, hipMemcpyDeviceToHost, st));
+  GZ_HIP(hipStreamSynchronize(st));
+  if (want_cc && ncc_out) *ncc_out = hsmall.p[0];
+  if (hflags.p[kFlagBadInput]) {
+    launch_fill_u32(flags.p + kFlagBadInput, 0, 1, st);
+    GZ_HIP(hipStreamSynchronize(st));
+    throw GzError(GZ_ERR_INVALID_ARG, "ingested device pairs named a vertex >= num_vertices; those pairs were skipped");
+  }
+  return cached;
 }
 
 // calc_spanning_forest, src/cc_sketch_alg.cpp:559-568: every root linked under another records the
@@ -1025,8 +1066,8 @@ int gz_alg_set_profiling(gz_alg* a, int enable) {
     GzAlg& A = *a->a;
     A.flush_all();
     if (!A.prof_bytes.p) {
-      A.prof_bytes.alloc(1);
-      gz::launch_fill_u64(A.prof_bytes.p, 0, 1, A.st);
+      A.prof_bytes.alloc(gz::kProfSlots);
+      gz::launch_fill_u64(A.prof_bytes.p, 0, gz::kProfSlots, A.st);
       GZ_HIP(hipStreamSynchronize(A.st));
     }
     A.prof = enable != 0;

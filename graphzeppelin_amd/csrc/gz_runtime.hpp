@@ -142,7 +142,7 @@ class GzAlg {
   DsuDev dsu{};
   // Borůvka round buffers
   DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members, r_totals, mod;
-  DevBuf<uint64_t> r_part[2];
+  DevBuf<uint64_t> r_part[2], r_samp;
   DevBuf<uint32_t> r_proot[2];
   RoundBufs rb{};
   // device stream ingest (pending pairs -> grouping -> tiles)
@@ -222,6 +222,9 @@ class GzAlg {
   void prep_query(int code);
   uint32_t connected_components(uint32_t* parent_out);
   void point_queries(const uint32_t* us, const uint32_t* vs, size_t count, uint8_t* out);
+  // Answers a group of queries against one state; returns whether the cached DSU answered them.
+  bool answer_queries(const uint32_t* us, const uint32_t* vs, size_t npoint, bool want_cc, uint32_t* ncc_out,
+                      uint8_t* point_out);
   size_t spanning_forest(uint32_t* edges, size_t cap);
   void dump(uint32_t v0, uint32_t count, void* out);
   void load_packed(uint32_t v0, uint32_t count, const void* in);

@@ -27,10 +27,6 @@ static double secs(Clock::time_point a, Clock::time_point b) { return std::chron
 // Answer the queries reqs[i, j) of one instance against one consistent state.
 static void answer_group(GzAlg& A, const gz_request* reqs, size_t i, size_t j, uint32_t* answers,
                          gz_serve_stats& st) {
-  const bool cached = A.has_cached_query(GZ_CONNECTIVITY);
-  if (cached) ++st.cached_answers;
-  else ++st.boruvka_runs;
-  ++st.query_groups;
   bool any_cc = false;
   std::vector<uint32_t> us, vs;
   std::vector<size_t> where;
@@ -44,9 +40,11 @@ static void answer_group(GzAlg& A, const gz_request* reqs, size_t i, size_t j, u
     }
   }
   uint32_t ncc = 0;
-  if (any_cc) ncc = A.connected_components(nullptr);
   std::vector<uint8_t> out(us.size());
-  if (!us.empty()) A.point_queries(us.data(), vs.data(), us.size(), out.data());
+  const bool cached = A.answer_queries(us.data(), vs.data(), us.size(), any_cc, &ncc, out.data());
+  if (cached) ++st.cached_answers;
+  else ++st.boruvka_runs;
+  ++st.query_groups;
   for (size_t k = i; k < j; ++k)
     if (reqs[k].kind == GZ_REQ_CC && answers) answers[k] = ncc;
   for (size_t q = 0; q < where.size(); ++q)
