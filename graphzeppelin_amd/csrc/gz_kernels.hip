@@ -206,9 +206,9 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
 // (src/cc_sketch_alg.cpp:106-118, src/sketch.cpp:149-154).
 //
 // One workgroup per tile (<= tile-size destinations of one vertex).  The tile's delta sketch
-// lives in LDS; lane l starts at column l % C and walks the columns cyclically, so the lanes of a
-// wave XOR into different columns at the same time (column pitch Dp = 1 mod 16 spreads them over
-// the LDS banks) instead of all hitting depth 0 of one column.  The deterministic bucket is kept
+// lives in LDS; each thread hashes (update, column) pairs, so consecutive lanes XOR into different
+// columns at the same time (column pitch Dp = 1 mod 16 spreads them over the LDS banks) instead of
+// all hitting depth 0 of one column.  The deterministic bucket is kept
 // in registers per lane and reduced once per tile.  The write-back is a coalesced
 // read-xor-write of the vertex's bucket planes, or device-scope atomic XOR when the vertex is
 // split across tiles of the same launch.  Workgroups walk the tiles grid-stride, so a persistent
@@ -236,8 +236,11 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   for (uint32_t i = threadIdx.x; i < lds_n; i += BLOCK) { s_a[i] = 0; s_g[i] = 0; }
   const uint32_t ntiles = ntiles_dev ? *ntiles_dev : ntiles_host;
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t start_col = lane % C;
   const uint64_t depth_cap = 1ull << D;
+  // (update, column) pairs are spread over the threads: thread t starts at pair t = i0 * C + c0
+  // and advances BLOCK pairs at a time (di updates and dc columns, with a carry).
+  const uint32_t i0 = threadIdx.x / C, c0 = threadIdx.x % C;
+  const uint32_t di = BLOCK / C, dc = BLOCK % C;
   // Tiles are taken grid-stride (t, t + gridDim.x, ...): no shared counter, so no device-wide
   // atomic on one hot address per tile.
   if (threadIdx.x == 0) s_tile = blockIdx.x;
@@ -250,22 +253,30 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
 
     uint64_t det_a = 0;
     G det_g = 0;
-    for (uint32_t i = threadIdx.x; i < tl.count; i += BLOCK) {
+    // One thread per (update, column) pair, so a tile of a few updates still keeps every lane
+    // busy; the column-0 thread of each update also folds it into the deterministic bucket.
+    const uint32_t npairs = tl.count * C;
+    uint32_t i = i0, c = c0;
+    for (uint32_t pr = threadIdx.x; pr < npairs; pr += BLOCK) {
       const uint64_t idx = RAW ? raw_idx[tl.begin + i] : concat_pairing(tl.vertex, dsts[tl.begin + i]);
       const uint64_t pre = xxh3_prep(idx);
       const G chk = (G)xxh3_mix(pre, g.chk_bf);
-      det_a ^= idx;
-      det_g ^= chk;
-      uint32_t c = start_col;
-      for (uint32_t s = 0; s < C; ++s) {
-        const uint64_t h = xxh3_mix(pre, s_bf[c]);
-        const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);
-        if (depth < D) {
-          const uint32_t li = c * Dp + depth;
-          lds_xor(s_a + li, idx);
-          lds_xor(s_g + li, chk);
-        }
-        c = (c + 1 == C) ? 0 : c + 1;
+      if (c == 0) {
+        det_a ^= idx;
+        det_g ^= chk;
+      }
+      const uint64_t h = xxh3_mix(pre, s_bf[c]);
+      const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);
+      if (depth < D) {
+        const uint32_t li = c * Dp + depth;
+        lds_xor(s_a + li, idx);
+        lds_xor(s_g + li, chk);
+      }
+      i += di;
+      c += dc;
+      if (c >= C) {
+        c -= C;
+        ++i;
       }
     }
     det_a = wave_xor_u64(det_a);
@@ -607,6 +618,7 @@ k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ 
 constexpr int kAggBlock = 1024;
 constexpr uint32_t kAggTable = 2048;  // >= 2 * kAggBlock, power of two
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot, no giant)
 
 __device__ inline uint32_t lds_table_slot(uint32_t* keys, uint32_t key) {
   uint32_t h = (key * 2654435761u) & (kAggTable - 1);
@@ -619,14 +631,17 @@ __device__ inline uint32_t lds_table_slot(uint32_t* keys, uint32_t key) {
 
 __global__ void __launch_bounds__(kAggBlock)
 k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root, uint32_t* __restrict__ cnt,
-                uint32_t* __restrict__ seg_cursor, const uint32_t* __restrict__ prev_mod) {
+                uint32_t* __restrict__ seg_cursor, uint32_t* __restrict__ giant, const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable];
   for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock) {
     t_key[i] = kEmptyKey;
     t_cnt[i] = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *seg_cursor = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *seg_cursor = 0;
+    *giant = kNoRoot;
+  }
   __syncthreads();
   const uint32_t lane = lane_id();
   const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
@@ -654,13 +669,18 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
 
 __global__ void __launch_bounds__(kAggBlock)
 k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-           uint32_t* __restrict__ seg_cursor, const uint32_t* __restrict__ prev_mod) {
+           uint32_t* __restrict__ seg_cursor, uint32_t* __restrict__ giant, uint32_t giant_ok,
+           const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   __shared__ uint64_t wsum[kAggBlock / kWave];
   __shared__ uint32_t s_base;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
   const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
-  const uint32_t c = v < n ? cnt[v] : 0;
+  uint32_t c = v < n ? cnt[v] : 0;
+  if (giant_ok && 2ull * c > n) {  // at most one root holds more than half of the vertices
+    *giant = v;
+    c = 0;
+  }
   const uint64_t incl = wave_incl_scan_u64(c);
   if (lane == kWave - 1) wsum[wid] = incl;
   __syncthreads();
@@ -680,7 +700,7 @@ k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ 
 
 __global__ void __launch_bounds__(kAggBlock)
 k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
-             uint32_t* __restrict__ members, const uint32_t* __restrict__ prev_mod) {
+             uint32_t* __restrict__ members, const uint32_t* __restrict__ giant, const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable], t_base[kAggTable];
   for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock) {
@@ -691,8 +711,9 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
   const uint32_t lane = lane_id();
   const uint64_t lt = (1ull << lane) - 1;
   const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool active = v < n;
-  const uint32_t r = active ? root[v] : 0;
+  const uint32_t gr = *giant;
+  const uint32_t r = v < n ? root[v] : 0;
+  const bool active = v < n && r != gr;  // the giant supernode's members are not regrouped
   uint32_t my_slot = 0, my_pos = 0;
   uint64_t todo = __ballot(active);
   while (todo) {
@@ -724,15 +745,36 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
 // a wave owns 64 consecutive entries, lane l < D accumulates depth l of the column and lane D the
 // deterministic bucket.  A supernode whose entries all lie inside the wave is sampled from
 // registers; its sample record goes to samp[root] for k_apply_samples.  A supernode that crosses
-// the wave's first (head) or last (tail) boundary leaves its
-// partial XOR in out[2w] / out[2w+1] under its root; unused outputs get root kNoRoot and zeros.
-// The next level reduces those 2*waves entries the same way, so a supernode spanning many waves is
-// combined without any atomics; the last level has a single wave.
-constexpr uint32_t kNoRoot = 0xFFFFFFFFu;
+// the wave's first (head) or last (tail) boundary leaves its partial XOR in out[2w] / out[2w+1]
+// under its root; unused outputs get root kNoRoot and zeros.  The next level reduces those entries
+// the same way, so a supernode spanning many waves is combined without any atomics; the last level
+// has a single wave.
+//
+// Giant supernode.  Every edge update XORs the same buckets into both endpoints' sketches, so the
+// XOR of all vertex sketches is zero and a supernode's sum equals the XOR of every vertex outside
+// it.  When one root holds more than half of the vertices (and the sketches are known to sum to
+// zero) its members are not regrouped: each level-1 wave also XORs all of its entries into a
+// total, the totals form one extra segment [2W, 3W) under the giant's root at level 2, and the
+// giant is sampled from that sum -- the same bytes the member-wise merge would give.
+//
+// Entry counts are on the device: level 1 has e1 = (entries regrouped) entries; level 2 has
+// 2*W1 (+ W1 totals with a giant) with W1 = ceil(e1/64); later levels 2*ceil(e/64).  A level whose
+// predecessor had a single wave is a no-op.
+__device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, uint32_t* e) {
+  uint32_t cur = e1;
+  for (uint32_t l = 1; l < level; ++l) {
+    const uint32_t w = (cur + kWave - 1) / kWave;
+    if (w <= 1) return false;
+    cur = 2 * w + ((l == 1 && giant) ? w : 0);
+  }
+  *e = cur;
+  return true;
+}
 
 template <class G, bool FIRST>
 __global__ void __launch_bounds__(256)
-k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
+k_bv_reduce(uint32_t level, const uint32_t* __restrict__ seg_total, const uint32_t* __restrict__ giant_p,
+            const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
             const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint32_t round,
             const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
             uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
@@ -742,11 +784,21 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
   if (FIRST && zero_cnt) {  // clear the count buffer the next round will use
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) zero_cnt[i] = 0;
   }
+  const uint32_t giant = *giant_p;
+  const bool has_giant = giant != kNoRoot;
+  const uint32_t e1 = *seg_total;
+  uint32_t n_entries;
+  if (!level_entries(e1, has_giant, level, &n_entries)) return;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (FIRST && has_giant && e1 == 0 && wave == 0) {  // every vertex is in the giant: its sum is zero
+    if (lane == 0) samp[giant] = kSampZero;
+    return;
+  }
   const uint32_t p0 = wave * kWave;
   if (p0 >= n_entries) return;
   const uint32_t p1 = (p0 + kWave < n_entries) ? p0 + kWave : n_entries;
   const uint32_t len = p1 - p0;
+  const uint32_t waves_here = (n_entries + kWave - 1) / kWave;
 
   auto entry_root = [&](uint32_t p) -> uint32_t { return FIRST ? vroot[members[p]] : in_root[p]; };
   uint32_t m = 0, rt = kNoRoot;
@@ -768,16 +820,18 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
   uint64_t* out_a = out_part + (size_t)(2 * wave) * 2 * kWave;  // entry layout: alpha[64] then gamma[64]
   bool wrote_head = false, wrote_tail = false;
 
+  auto sample_to = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g) {
+    const uint64_t det_a = rd_lane_u64(acc_a, g.D);
+    const G det_g = (G)rd_lane_u64(acc_g, g.D);
+    uint64_t idx;
+    const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
+    if (lane == 0) samp[R] = sample_record(code, idx, g.n, flags);
+  };
   auto emit = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g, uint32_t seg_start, uint32_t seg_end) {
     const bool started_before = seg_start == p0 && head_open;
     const bool continues = seg_end == p1 && tail_open;
     if (!started_before && !continues) {
-      if (R == kNoRoot) return;
-      const uint64_t det_a = rd_lane_u64(acc_a, g.D);
-      const G det_g = (G)rd_lane_u64(acc_g, g.D);
-      uint64_t idx;
-      const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
-      if (lane == 0) samp[R] = sample_record(code, idx, g.n, flags);
+      if (R != kNoRoot) sample_to(R, acc_a, acc_g);
       return;
     }
     // Partial of a spanning supernode: the head slot if it started before this wave, else the tail
@@ -799,7 +853,7 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
     }
   };
 
-  uint64_t acc_a = 0, acc_g = 0;
+  uint64_t acc_a = 0, acc_g = 0, all_a = 0, all_g = 0;
   uint32_t cur = first_root, seg_start = p0;
   constexpr uint32_t U = FIRST ? 8 : 16;  // loads in flight per lane
   for (uint32_t j = 0; j < len; j += U) {
@@ -833,10 +887,24 @@ k_bv_reduce(uint32_t n_entries, const uint32_t* __restrict__ members, const uint
         }
         acc_a ^= va[k];
         acc_g ^= vg[k];
+        if (FIRST) {
+          all_a ^= va[k];
+          all_g ^= vg[k];
+        }
       }
     }
   }
   emit(cur, acc_a, acc_g, seg_start, p1);
+  if (FIRST && has_giant) {
+    if (waves_here == 1) {
+      sample_to(giant, all_a, all_g);  // single wave: its total is the giant's whole sum
+    } else {
+      uint64_t* t = out_part + (size_t)(2 * waves_here + wave) * 2 * kWave;
+      t[lane] = all_a;
+      t[kWave + lane] = all_g;
+      if (lane == 0) out_root[2 * waves_here + wave] = giant;
+    }
+  }
   // unused output slots carry no root and no value
   if (!wrote_head) {
     out_a[lane] = 0;
@@ -980,6 +1048,27 @@ __global__ void k_packed(uint64_t* __restrict__ alpha, G* __restrict__ gamma, Ge
 //   D = sum over vertices v, buckets b of fmix64(x + pos), pos = v*nb + b,
 //   x = alpha ^ fmix64(gamma ^ pos * 0x9E3779B97F4A7C15)   (all mod 2^64)
 // A sum is independent of reduction order, so the value is deterministic.
+// Bucket-wise XOR of every vertex sketch into out_a / out_g (nb entries each, zero on entry): one
+// thread per (bucket, chunk of kSumChunk vertices), one device-scope atomic per thread.
+constexpr uint32_t kSumChunk = 1024;
+template <class G>
+__global__ void __launch_bounds__(256)
+k_sum_all(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint64_t* __restrict__ out_a,
+          G* __restrict__ out_g) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.nb) return;
+  const uint32_t v0 = blockIdx.y * kSumChunk;
+  const uint32_t v1 = v0 + kSumChunk < g.n ? v0 + kSumChunk : g.n;
+  uint64_t a = 0;
+  G gm = 0;
+  for (uint32_t v = v0; v < v1; ++v) {
+    a ^= alpha[(size_t)v * g.stride + b];
+    gm ^= gamma[(size_t)v * g.stride + b];
+  }
+  if (a) gbl_xor(out_a + b, a);
+  if (gm) gbl_xor(out_g + b, gm);
+}
+
 template <class G>
 __global__ void __launch_bounds__(256)
 k_digest(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, unsigned long long* __restrict__ out) {
@@ -1084,56 +1173,85 @@ void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, 
                      d.mod);
 }
 
+// Host bound on each level's entry count (the kernels compute the exact counts on the device):
+// the larger of the chain without a giant (e1 <= n) and with one (e1 < n/2, level 2 = 3 * W1).
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) {
+  uint32_t a = n, b = n / 2;
   size_t L = 0;
-  uint32_t e = n;
+  bool a_done = false, b_done = false;
   while (true) {
-    if (L < max_levels && entries_per_level) entries_per_level[L] = e;
+    if (L < max_levels && entries_per_level) entries_per_level[L] = std::max(a_done ? 0u : a, b_done ? 0u : b);
     ++L;
-    const uint32_t waves = (e + kWave - 1) / kWave;
-    if (waves <= 1) break;
-    e = 2 * waves;
+    const uint32_t wa = (a + kWave - 1) / kWave, wb = (b + kWave - 1) / kWave;
+    if (wa <= 1) a_done = true;
+    if (wb <= 1) b_done = true;
+    if (a_done && b_done) break;
+    a = 2 * wa;
+    b = 2 * wb + (L == 1 ? wb : 0);
   }
   return L;
 }
 
 void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
-                  const DsuDev& d, const RoundBufs& rb, hipStream_t s) {
+                  const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s) {
   const uint32_t n = g.n;
   const uint32_t* prev = d.mod + (round - 1);
   uint32_t* cur = d.mod + round;
   uint32_t* cnt = rb.cnt;  // zero on entry; the level-1 reduce clears it again for the next round
+  uint32_t* seg_total = rb.totals;
+  uint32_t* giant = rb.totals + 1;
   const uint32_t grid = (n + kAggBlock - 1) / kAggBlock;  // one vertex per thread
-  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, cnt, rb.totals, prev);
-  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, cnt, n, rb.off, rb.cursor, rb.totals, prev);
-  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, rb.members, prev);
+  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, cnt, seg_total, giant,
+                     prev);
+  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, cnt, n, rb.off, rb.cursor, seg_total, giant,
+                     giant_ok ? 1u : 0u, prev);
+  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, rb.members, giant, prev);
   uint32_t ent[32];
   const size_t L = round_levels(n, ent, 32);
   for (size_t l = 0; l < L; ++l) {
     const uint32_t e = ent[l];
     const uint32_t waves = (e + kWave - 1) / kWave;
-    const uint32_t blocks = (waves * kWave + 255) / 256;
+    const uint32_t blocks = std::max(1u, (waves * kWave + 255) / 256);
+    const uint32_t level = (uint32_t)l + 1;
     uint64_t* in_p = rb.part[(l + 1) & 1];
     uint32_t* in_r = rb.proot[(l + 1) & 1];
     uint64_t* out_p = rb.part[l & 1];
     uint32_t* out_r = rb.proot[l & 1];
     if (l == 0) {
       if (gbytes == 4)
-        hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, cnt);
+        hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
+                           rb.members, rb.root, alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r,
+                           rb.samp, d.flags, prev, cnt);
       else
-        hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, cnt);
+        hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
+                           rb.members, rb.root, alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r,
+                           rb.samp, d.flags, prev, cnt);
     } else {
       if (gbytes == 4)
-        hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, nullptr);
+        hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
+                           rb.members, rb.root, alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r,
+                           rb.samp, d.flags, prev, nullptr);
       else
-        hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, e, rb.members, rb.root, alpha,
-                           (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev, nullptr);
+        hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
+                           rb.members, rb.root, alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r,
+                           rb.samp, d.flags, prev, nullptr);
     }
   }
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, prev, cur);
+}
+
+void launch_sum_all(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out_a, void* out_g,
+                    hipStream_t s) {
+  launch_fill_u64(out_a, 0, g.nb, s);
+  if (gbytes == 4) launch_fill_u32((uint32_t*)out_g, 0, g.nb, s);
+  else launch_fill_u64((uint64_t*)out_g, 0, g.nb, s);
+  const dim3 grid((g.nb + 255) / 256, (g.n + kSumChunk - 1) / kSumChunk);
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_sum_all<uint32_t>, grid, dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, out_a,
+                       (uint32_t*)out_g);
+  else
+    hipLaunchKernelGGL(k_sum_all<uint64_t>, grid, dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, out_a,
+                       (uint64_t*)out_g);
 }
 
 void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s) {

@@ -82,22 +82,27 @@ struct RoundBufs {
   uint32_t* off;       // n      segment start per root
   uint32_t* cursor;    // n      scatter cursor per root
   uint32_t* members;   // n      vertices grouped by root
-  uint32_t* totals;    // [0]    segment allocation cursor
+  uint32_t* totals;    // [0] segment allocation cursor (= level-1 entries), [1] giant root or ~0
   uint64_t* part[2];   // partial-XOR entries of the multi-level reduce, 2*ceil(n/64) entries of 128 u64
   uint32_t* proot[2];  // root of each partial entry
   uint64_t* samp;      // n      sample record per supernode root
 };
-// Number of reduce levels for n entries (entries_per_level gets each level's entry count).
+// Number of reduce levels for n vertices (entries_per_level gets a bound on each level's entries).
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
 // Queue round `round` (>= 1); every kernel of it is a no-op when d.mod[round - 1] == 0.
+// giant_ok: the vertex sketches are known to XOR to zero, so a supernode holding more than half of
+// the vertices may be summed from its complement.
 void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
-                  const DsuDev& d, const RoundBufs& rb, hipStream_t s);
+                  const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
 void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,
                           hipStream_t s);
 
 // order-sensitive 64-bit digest of all n sketches (see k_digest)
 void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s);
+// Bucket-wise XOR of all n vertex sketches (out_a: nb u64, out_g: nb gamma words).
+void launch_sum_all(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out_a, void* out_g,
+                    hipStream_t s);
 
 // adds sum_v 4k + 2*min(S, k*per_update) over per-vertex counts k = cnt[v] to *out
 

@@ -310,6 +310,7 @@ void GzAlg::apply_update_batch(uint32_t src, const uint32_t* dsts, size_t k) {
   if (!k) return;
   if (k > 0xffffffffull) throw GzError(GZ_ERR_INVALID_ARG, "batch too large");
   std::lock_guard<std::mutex> lk(stage_mtx);
+  zero_sum = 0;  // one direction of an edge alone would break the zero sum
   if (!cfg.batched) {
     plain_launch(src, dsts, k);
     return;
@@ -430,6 +431,7 @@ void GzAlg::apply_raw(uint32_t src, const void* rawb) {
   GZ_HIP(hipMemcpyAsync(tmp.p, rawb, sketch_bytes(), hipMemcpyHostToDevice, st));
   launch_merge_packed(gb, alpha.p, gamma.p, geo, src, 1, tmp.p, st);
   GZ_HIP(hipStreamSynchronize(st));
+  zero_sum = 0;  // a raw delta on one vertex may break the zero sum
 }
 
 // ---- kernel timing ----
@@ -574,6 +576,27 @@ void GzAlg::ensure_answer() {
   boruvka();
 }
 
+// Edge updates keep the XOR of all vertex sketches at zero; raw bucket deltas and loaded images
+// may not, so after those the sum is checked once (one pass over the sketches).
+bool GzAlg::sketches_sum_to_zero() {
+  if (zero_sum == 0) {
+    DevBuf<uint64_t> sa, sg;
+    sa.alloc(geo.nb);
+    sg.alloc(geo.nb);
+    launch_sum_all(gb, alpha.p, gamma.p, geo, sa.p, sg.p, st);
+    std::vector<uint64_t> ha(geo.nb), hg(geo.nb);
+    GZ_HIP(hipMemcpyAsync(ha.data(), sa.p, geo.nb * 8, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipMemcpyAsync(hg.data(), sg.p, geo.nb * (size_t)gb, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    bool zero = true;
+    for (uint32_t b = 0; b < geo.nb && zero; ++b) zero = ha[b] == 0;
+    const uint8_t* gbytes = reinterpret_cast<const uint8_t*>(hg.data());
+    for (size_t i = 0; i < geo.nb * (size_t)gb && zero; ++i) zero = gbytes[i] == 0;
+    zero_sum = zero ? 1 : -1;
+  }
+  return zero_sum == 1;
+}
+
 // boruvka_emulation, src/cc_sketch_alg.cpp:464-513.  Rounds are queued in groups; every kernel
 // of round r is a no-op once round r-1 changed nothing, so the host reads the per-round flags once
 // per group instead of once per round.
@@ -581,6 +604,7 @@ void GzAlg::boruvka() {
   update_locked = true;
   try {
     const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
+    const bool giant_ok = sketches_sum_to_zero();
     launch_dsu_reset(dsu, st);
     launch_fill_u32(r_cnt.p, 0, n, st);
     if (prof) prof_begin(1, 0, false, 0);
@@ -591,7 +615,8 @@ void GzAlg::boruvka() {
     int64_t rounds = -1;
     while (true) {
       const uint32_t hi = std::min(next + group - 1, last_round);
-      for (uint32_t r = next; r <= hi && next <= last_round; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, st);
+      for (uint32_t r = next; r <= hi && next <= last_round; ++r)
+        launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
       GZ_HIP(hipGetLastError());
       const uint32_t upto = std::max(hi, next - 1);
       GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)(upto + 1) * 4, hipMemcpyDeviceToHost, st));
@@ -751,6 +776,7 @@ void GzAlg::load_packed(uint32_t v0, uint32_t count, const void* in) {
     launch_unpack(gb, alpha.p, gamma.p, geo, v0 + i, c, tmp.p, st);
     GZ_HIP(hipStreamSynchronize(st));
   }
+  zero_sum = 0;
 }
 
 }  // namespace gz

@@ -202,6 +202,10 @@ class GzAlg {
   std::atomic<bool> dsu_valid{true};
   std::atomic<bool> update_locked{false};
   size_t last_rounds = 0;
+  // Whether the vertex sketches XOR to zero (true for sketches built from edge updates only):
+  // 1 known zero, 0 unknown (checked at the next query), -1 known not zero.
+  std::atomic<int> zero_sum{1};
+  bool sketches_sum_to_zero();
   std::atomic<uint64_t> total_updates{0};
   std::mutex query_mtx;
 
