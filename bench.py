@@ -68,6 +68,32 @@ def prefix_trace(trace: np.ndarray, upto: int) -> np.ndarray:
     return np.array(out, dtype=trace.dtype)
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_update_tiles_chunglu_n262144_traffic.md")
+
+
+def workload_key(args) -> str:
+    return (f"n={args.n} updates={args.updates} del={args.del_frac} per_request={args.per_request} "
+            f"seed={args.seed} sketch_seed={args.sketch_seed} gamma={args.gamma_bytes} tile={args.tile} "
+            f"flush_threshold={args.flush_threshold}")
+
+
+def measured_traffic(args):
+    """HBM bytes per k_update_tiles launch from the committed rocprofv3 --pmc summary (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the same counter over a 1 GiB device copy), when it was measured on
+    this exact workload; otherwise None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            kv = dict(line.split(":", 1) for line in f if line.startswith(("workload_key:", "traffic_bytes_per_launch:")))
+    except OSError:
+        return None
+    if kv.get("workload_key", "").strip() != workload_key(args):
+        return None
+    try:
+        return float(kv["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def pct(x, q):
     return float(np.percentile(x, q)) if len(x) else None
 
@@ -90,6 +116,9 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=4_000_000, help="updates in the plain-form prefix")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy 1 GiB device-to-device 3 times (a plain copy of known "
+                         "size that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -147,6 +176,14 @@ def main() -> int:
                     prof_tot[key][f] += prof[key][f]
         log(f"step {k}: {dt:.3f}s  {st['updates'] / dt / 1e6:.1f} M updates/s  borůvka={st['boruvka_runs']} "
             f"cached={st['cached_answers']}")
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     qmask = trace["kind"] != 0
     qlat = np.concatenate([l[qmask] for l in lats]) * 1e3
     cc_lat = np.concatenate([l[trace["kind"] == 2] for l in lats]) * 1e3
@@ -192,7 +229,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args),
             "launches": up["launches"],
             "avg_launch_ms": up["total_ms"] / max(1, up["launches"]),
             "alg_bytes_per_launch": up["alg_bytes"] / max(1, up["launches"]),
