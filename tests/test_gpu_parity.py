@@ -268,3 +268,43 @@ def test_spanning_forest_is_a_forest_of_the_graph():
     assert set(streams.edge_ids(sf).tolist()) <= live
     c2, lab2 = oracle.exact_cc(400, sf)
     assert c2 == c and np.array_equal(lab2, lab)
+
+
+def test_majority_supernode_rounds_equal_oracle():
+    # a giant component holding most vertices plus stragglers: rounds after the first few sum the
+    # giant from its complement, which must give the oracle's partition and round count exactly
+    pairs, _ = streams.dynamic_erdos_stream(1500, 0.0012, 0.3, 0.2, 3, 21)  # 59 components, largest 1431
+    for gb in (4, 8):
+        ref = oracle.CCAlg(1500, 77, gamma_bytes=gb, eager=False)
+        ref.ingest_pairs(pairs)  # pre_insert without the eager DSU invalidates the cached answer
+        alg = gz.CCSketchAlg(1500, 77, gz.CCAlgConfiguration(gamma_bytes=gb, eager_dsu=False))
+        alg.ingest_pairs(pairs)
+        cc = alg.connected_components()
+        rparents, rc = ref.connected_components()
+        assert alg.last_query_rounds == ref.last_query_rounds
+        assert np.array_equal(cc.canonical_labels(), oracle.canonical(rparents))
+        c, lab = oracle.exact_cc(1500, pairs)
+        assert cc.size() == c and np.array_equal(cc.canonical_labels(), lab)
+
+
+def test_one_sided_batches_and_raw_deltas_query_like_the_oracle():
+    # apply_update_batch with one direction only and raw bucket deltas leave sketches whose XOR is
+    # not zero; queries must then sum every supernode member-wise and still match the oracle
+    pairs, _ = streams.erdos_renyi_churn_stream(700, 0.01, 5)
+    ref = oracle.CCAlg(700, 31, eager=False)
+    alg = gz.CCSketchAlg(700, 31, gz.CCAlgConfiguration(eager_dsu=False))
+    for v, ds in _directed_batches(pairs, 64):
+        keep = ds[(ds + v) % 7 != 0]  # drop some directions
+        if keep.size:
+            ref.apply_update_batch(0, v, keep)
+            alg.apply_update_batch(0, v, keep)
+    raw = ref.dump(3, 1).tobytes()
+    ref.apply_raw_buckets_update(11, raw)
+    alg.apply_raw_buckets_update(11, raw)
+    ref.pre_insert(0, 1)  # without the eager DSU this invalidates the cached (empty) answer
+    alg.pre_insert(0, 1)
+    assert np.array_equal(alg.dump_sketches(), ref.dump())
+    cc = alg.connected_components()
+    rparents, rc = ref.connected_components()
+    assert alg.last_query_rounds == ref.last_query_rounds
+    assert np.array_equal(cc.canonical_labels(), oracle.canonical(rparents))
