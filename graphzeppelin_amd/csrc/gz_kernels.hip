@@ -566,18 +566,36 @@ template <class G>
 __global__ void __launch_bounds__(256)
 k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint64_t* __restrict__ samp,
                 uint32_t* __restrict__ flags) {
+  constexpr uint32_t V = 4;  // vertices per wave iteration: all their loads are issued first
   const uint32_t lane = lane_id();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
-  for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < g.n; v += waves) {
-    const size_t base = (size_t)v * g.stride;
-    const uint64_t det_a = alpha[base + g.nb - 1];
-    const G det_g = gamma[base + g.nb - 1];
-    const bool valid = lane < g.D;
-    const uint64_t a = valid ? alpha[base + lane] : 0;
-    const G gm = valid ? gamma[base + lane] : 0;
-    uint64_t idx;
-    const int code = wave_sample<G>(det_a, det_g, a, gm, valid, g.chk_bf, &idx);
-    if (lane == 0) samp[v] = sample_record(code, idx, g.n, flags);
+  const bool valid = lane < g.D;
+  for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * V; v0 < g.n; v0 += waves * V) {
+    uint64_t det_a[V], a[V];
+    G det_g[V], gm[V];
+#pragma unroll
+    for (uint32_t k = 0; k < V; ++k) {
+      det_a[k] = 0;
+      det_g[k] = 0;
+      a[k] = 0;
+      gm[k] = 0;
+      if (v0 + k < g.n) {
+        const size_t base = (size_t)(v0 + k) * g.stride;
+        det_a[k] = alpha[base + g.nb - 1];
+        det_g[k] = gamma[base + g.nb - 1];
+        if (valid) {
+          a[k] = alpha[base + lane];
+          gm[k] = gamma[base + lane];
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < V; ++k) {
+      if (v0 + k >= g.n) break;
+      uint64_t idx;
+      const int code = wave_sample<G>(det_a[k], det_g[k], a[k], gm[k], valid, g.chk_bf, &idx);
+      if (lane == 0) samp[v0 + k] = sample_record(code, idx, g.n, flags);
+    }
   }
 }
 
@@ -1164,7 +1182,7 @@ void launch_dsu_reset(const DsuDev& d, hipStream_t s) {
 
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
                           uint64_t* samp, hipStream_t s) {
-  const uint32_t grid = grid_for((size_t)g.n * kWave, 256, 16384);
+  const uint32_t grid = grid_for((size_t)(g.n + 3) / 4 * kWave, 256, 16384);
   if (gbytes == 4)
     hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, samp, d.flags);
   else
