@@ -45,7 +45,7 @@ _SIGS = {
     "or_alg_sketch_bytes": (_sz, [_vp]),
     "or_alg_desired_updates_per_batch": (_sz, [_vp]),
     "or_alg_allocate_worker_memory": (None, [_vp, _sz]),
-    "or_alg_apply_update_batch": (None, [_vp, ctypes.c_int, _u32, _vp, _sz]),
+    "or_alg_apply_update_batch": (ctypes.c_int, [_vp, ctypes.c_int, _u32, _vp, _sz]),
     "or_alg_apply_raw": (None, [_vp, _u32, _vp]),
     "or_alg_pre_insert": (None, [_vp, _u32, _u32]),
     "or_alg_update": (None, [_vp, _u32, _u32]),
@@ -191,7 +191,8 @@ class CCAlg:
 
     def apply_update_batch(self, thr, src, dsts):
         a = np.ascontiguousarray(np.asarray(dsts, dtype=np.uint32))
-        lib().or_alg_apply_update_batch(self._h, thr, src, _p(a), a.size)
+        if lib().or_alg_apply_update_batch(self._h, thr, src, _p(a), a.size):
+            raise ValueError(f"worker {thr} has no delta sketch: call allocate_worker_memory first")
 
     def apply_raw_buckets_update(self, src, raw):
         b = np.frombuffer(bytes(raw), dtype=np.uint8)

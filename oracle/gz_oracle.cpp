@@ -593,9 +593,12 @@ size_t or_alg_desired_updates_per_batch(void* p) {
   auto* h = (OrAlg*)p; size_t r = 0; DISPATCH_ALG(h, r = A.desired_updates_per_batch()); return r;
 }
 void or_alg_allocate_worker_memory(void* p, size_t w) { auto* h = (OrAlg*)p; DISPATCH_ALG(h, A.allocate_worker_memory(w)); }
-void or_alg_apply_update_batch(void* p, int thr, uint32_t src, const uint32_t* dsts, size_t k) {
+// Returns 1 when thr has no worker delta (allocate_worker_memory was not called for it).
+int or_alg_apply_update_batch(void* p, int thr, uint32_t src, const uint32_t* dsts, size_t k) {
   auto* h = (OrAlg*)p;
-  DISPATCH_ALG(h, A.apply_update_batch(thr, src, dsts, k));
+  int bad = 0;
+  DISPATCH_ALG(h, if (thr < 0 || (size_t)thr >= A.delta.size()) bad = 1; else A.apply_update_batch(thr, src, dsts, k));
+  return bad;
 }
 void or_alg_apply_raw(void* p, uint32_t src, const void* raw) {
   auto* h = (OrAlg*)p;

@@ -293,6 +293,8 @@ def test_one_sided_batches_and_raw_deltas_query_like_the_oracle():
     pairs, _ = streams.erdos_renyi_churn_stream(700, 0.01, 5)
     ref = oracle.CCAlg(700, 31, eager=False)
     alg = gz.CCSketchAlg(700, 31, gz.CCAlgConfiguration(eager_dsu=False))
+    ref.allocate_worker_memory(1)
+    alg.allocate_worker_memory(1)
     for v, ds in _directed_batches(pairs, 64):
         keep = ds[(ds + v) % 7 != 0]  # drop some directions
         if keep.size:
