@@ -602,25 +602,52 @@ k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma,
 // The merges of a round (sample_supernode, src/cc_sketch_alg.cpp:137-166), one thread per
 // supernode: every vertex that was a root when the round began (root == nullptr: every vertex, as
 // in round 0) acts on its sample record.  The round is modified by a FAIL or a merge.
+constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot, no giant)
+
+// Round state words (RoundBufs::totals).  A query starts with kStGiantFirst = kNoRoot.
+//   kStSeg + (r & 1)  entries regrouped in round r (the level-1 entry count, and the length of the
+//                     active list of round r + 1)
+//   kStGiantRound     root of this round's giant supernode, or kNoRoot
+//   kStGiantFirst     the root elected giant first in this query (any member identifies the giant,
+//                     which only grows), or kNoRoot
+//   kStActMode        1 when this round walks the active list instead of every vertex
+// Once a giant exists, the active list of round r is the member list regrouped in round r - 1:
+// every vertex outside the giant.  Vertices that joined the giant are dropped as they are met.
+constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStWords = 8;
+
+// i-th vertex of this round's walk: the active list, or every vertex.
+__device__ inline uint32_t walk_count(const uint32_t* st, uint32_t round, uint32_t n, bool act) {
+  return act ? st[kStSeg + ((round - 1) & 1)] : n;
+}
+
+__device__ inline bool apply_sample(const DsuDev& d, uint64_t s) {
+  if (s == kSampZero) return false;
+  if (s == kSampFail) return true;
+  const uint32_t a = (uint32_t)(s >> 32), b = (uint32_t)s;
+  const uint32_t child = dsu_link(d.parent, a, b, d.prio_seed);
+  if (child == kNoLink) return false;
+  sf_record(d, child, a, b);
+  return true;
+}
+
+// st == nullptr (round 0): every vertex is a root.  Otherwise the round's walk (every vertex, or
+// the active list plus the giant's root, which the list does not hold).
 __global__ void __launch_bounds__(256)
 k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ root, DsuDev d,
+                const uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
                 const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod) {
   if (prev_mod && !ld_acq(prev_mod)) return;
+  const bool act = st && st[kStActMode] != 0;
+  const uint32_t gr = act ? st[kStGiantRound] : kNoRoot;
+  const uint32_t count = st ? walk_count(st, round, d.n, act) : d.n;
   bool modified = false;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += gridDim.x * blockDim.x) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const uint32_t v = act ? act_list[i] : i;
     if (root && root[v] != v) continue;
-    const uint64_t s = samp[v];
-    if (s == kSampZero) continue;
-    if (s == kSampFail) {
-      modified = true;
-      continue;
-    }
-    const uint32_t a = (uint32_t)(s >> 32), b = (uint32_t)s;
-    const uint32_t child = dsu_link(d.parent, a, b, d.prio_seed);
-    if (child == kNoLink) continue;
-    sf_record(d, child, a, b);
-    modified = true;
+    if (v == gr) continue;  // handled once below
+    if (apply_sample(d, samp[v])) modified = true;
   }
+  if (gr != kNoRoot && blockIdx.x == 0 && threadIdx.x == 0 && apply_sample(d, samp[gr])) modified = true;
   if (__syncthreads_or(modified) && threadIdx.x == 0) *cur_mod = 1;  // one plain store per block
 }
 
@@ -636,7 +663,6 @@ k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ 
 constexpr int kAggBlock = 1024;
 constexpr uint32_t kAggTable = 2048;  // >= 2 * kAggBlock, power of two
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
-constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot, no giant)
 
 __device__ inline uint32_t lds_table_slot(uint32_t* keys, uint32_t key) {
   uint32_t h = (key * 2654435761u) & (kAggTable - 1);
@@ -649,29 +675,36 @@ __device__ inline uint32_t lds_table_slot(uint32_t* keys, uint32_t key) {
 
 __global__ void __launch_bounds__(kAggBlock)
 k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root, uint32_t* __restrict__ cnt,
-                uint32_t* __restrict__ seg_cursor, uint32_t* __restrict__ giant, const uint32_t* __restrict__ prev_mod) {
+                uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
+                const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable];
   for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock) {
     t_key[i] = kEmptyKey;
     t_cnt[i] = 0;
   }
+  const uint32_t gfirst = st[kStGiantFirst];
+  const bool act = gfirst != kNoRoot;
+  const uint32_t gcur = act ? dsu_find_stable(parent, gfirst) : kNoRoot;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *seg_cursor = 0;
-    *giant = kNoRoot;
+    st[kStSeg + (round & 1)] = 0;
+    st[kStGiantRound] = gcur;
+    st[kStActMode] = act ? 1u : 0u;
   }
   __syncthreads();
   const uint32_t lane = lane_id();
-  const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool active = v < n;
+  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool in_walk = i < walk_count(st, round, n, act);
+  const uint32_t v = in_walk ? (act ? act_list[i] : i) : 0;
   uint32_t r = 0;
-  if (active) {
+  if (in_walk) {
     r = dsu_find_stable(parent, v);
     root[v] = r;
   }
   __syncthreads();
-  // full path compression: every vertex now points at its root (no links happen in this kernel)
-  if (active) parent[v] = r;
+  // full path compression: every walked vertex now points at its root (no links in this kernel)
+  if (in_walk) parent[v] = r;
+  const bool active = in_walk && r != gcur;  // members of the giant are not counted
   uint64_t todo = __ballot(active);
   while (todo) {
     const int leader = __builtin_ctzll(todo);
@@ -681,22 +714,30 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
     todo &= ~same;
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock)
-    if (t_key[i] != kEmptyKey) atomicAdd(cnt + t_key[i], t_cnt[i]);
+  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
+    if (t_key[k] != kEmptyKey) atomicAdd(cnt + t_key[k], t_cnt[k]);
 }
 
+// Segments for the roots met in this round's walk (every non-giant root is a member of its own
+// supernode, so it is walked).  cnt is cleared here, its last reader.  Without a giant yet, a root
+// holding more than half of the vertices is elected: at most one can.
 __global__ void __launch_bounds__(kAggBlock)
-k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
-           uint32_t* __restrict__ seg_cursor, uint32_t* __restrict__ giant, uint32_t giant_ok,
+k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
+           uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list, uint32_t giant_ok,
            const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   __shared__ uint64_t wsum[kAggBlock / kWave];
   __shared__ uint32_t s_base;
+  const bool act = st[kStActMode] != 0;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
-  const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
-  uint32_t c = v < n ? cnt[v] : 0;
-  if (giant_ok && 2ull * c > n) {  // at most one root holds more than half of the vertices
-    *giant = v;
+  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool in_walk = i < walk_count(st, round, n, act);
+  const uint32_t v = in_walk ? (act ? act_list[i] : i) : 0;
+  uint32_t c = in_walk ? cnt[v] : 0;
+  if (c) cnt[v] = 0;
+  if (!act && giant_ok && 2ull * c > n) {
+    st[kStGiantFirst] = v;
+    st[kStGiantRound] = v;
     c = 0;
   }
   const uint64_t incl = wave_incl_scan_u64(c);
@@ -707,7 +748,7 @@ k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ 
     if (w < wid) pre += wsum[w];
     tot += wsum[w];
   }
-  if (threadIdx.x == 0) s_base = tot ? atomicAdd(seg_cursor, (uint32_t)tot) : 0;
+  if (threadIdx.x == 0) s_base = tot ? atomicAdd(st + kStSeg + (round & 1), (uint32_t)tot) : 0;
   __syncthreads();
   if (c) {
     const uint32_t o = s_base + (uint32_t)(pre + incl - c);
@@ -718,20 +759,24 @@ k_bv_alloc(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ 
 
 __global__ void __launch_bounds__(kAggBlock)
 k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
-             uint32_t* __restrict__ members, const uint32_t* __restrict__ giant, const uint32_t* __restrict__ prev_mod) {
+             uint32_t* __restrict__ members, const uint32_t* __restrict__ st, uint32_t round,
+             const uint32_t* __restrict__ act_list, const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable], t_base[kAggTable];
-  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock) {
-    t_key[i] = kEmptyKey;
-    t_cnt[i] = 0;
+  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock) {
+    t_key[k] = kEmptyKey;
+    t_cnt[k] = 0;
   }
   __syncthreads();
+  const bool act = st[kStActMode] != 0;
+  const uint32_t gr = st[kStGiantRound];
   const uint32_t lane = lane_id();
   const uint64_t lt = (1ull << lane) - 1;
-  const uint32_t v = blockIdx.x * kAggBlock + threadIdx.x;
-  const uint32_t gr = *giant;
-  const uint32_t r = v < n ? root[v] : 0;
-  const bool active = v < n && r != gr;  // the giant supernode's members are not regrouped
+  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool in_walk = i < walk_count(st, round, n, act);
+  const uint32_t v = in_walk ? (act ? act_list[i] : i) : 0;
+  const uint32_t r = in_walk ? root[v] : 0;
+  const bool active = in_walk && r != gr;  // the giant supernode's members are not regrouped
   uint32_t my_slot = 0, my_pos = 0;
   uint64_t todo = __ballot(active);
   while (todo) {
@@ -752,8 +797,8 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
     todo &= ~same;
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock)
-    if (t_key[i] != kEmptyKey) t_base[i] = atomicAdd(cursor + t_key[i], t_cnt[i]);
+  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
+    if (t_key[k] != kEmptyKey) t_base[k] = atomicAdd(cursor + t_key[k], t_cnt[k]);
   __syncthreads();
   if (active) members[t_base[my_slot] + my_pos] = v;
 }
@@ -791,20 +836,16 @@ __device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, ui
 
 template <class G, bool FIRST>
 __global__ void __launch_bounds__(256)
-k_bv_reduce(uint32_t level, const uint32_t* __restrict__ seg_total, const uint32_t* __restrict__ giant_p,
-            const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
-            const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint32_t round,
-            const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
+k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
+            const uint32_t* __restrict__ vroot, const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g,
+            uint32_t round, const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
             uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
-            uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ zero_cnt) {
+            uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   const uint32_t lane = lane_id();
-  if (FIRST && zero_cnt) {  // clear the count buffer the next round will use
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) zero_cnt[i] = 0;
-  }
-  const uint32_t giant = *giant_p;
+  const uint32_t giant = st[kStGiantRound];
   const bool has_giant = giant != kNoRoot;
-  const uint32_t e1 = *seg_total;
+  const uint32_t e1 = st[kStSeg + (round & 1)];
   uint32_t n_entries;
   if (!level_entries(e1, has_giant, level, &n_entries)) return;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -1187,8 +1228,8 @@ void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, 
     hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, samp, d.flags);
   else
     hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, samp, d.flags);
-  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr,
-                     d.mod);
+  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr, 0u,
+                     nullptr, nullptr, d.mod);
 }
 
 // Host bound on each level's entry count (the kernels compute the exact counts on the device):
@@ -1215,15 +1256,16 @@ void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void*
   const uint32_t n = g.n;
   const uint32_t* prev = d.mod + (round - 1);
   uint32_t* cur = d.mod + round;
-  uint32_t* cnt = rb.cnt;  // zero on entry; the level-1 reduce clears it again for the next round
-  uint32_t* seg_total = rb.totals;
-  uint32_t* giant = rb.totals + 1;
-  const uint32_t grid = (n + kAggBlock - 1) / kAggBlock;  // one vertex per thread
-  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, cnt, seg_total, giant,
-                     prev);
-  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, cnt, n, rb.off, rb.cursor, seg_total, giant,
+  uint32_t* st = rb.totals;
+  const uint32_t* act_list = rb.members[(round - 1) & 1];  // read only in active-list mode
+  uint32_t* members = rb.members[round & 1];
+  const uint32_t grid = (n + kAggBlock - 1) / kAggBlock;  // one walked vertex per thread
+  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, rb.cnt, st, round,
+                     act_list, prev);
+  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, rb.cnt, n, rb.off, rb.cursor, st, round, act_list,
                      giant_ok ? 1u : 0u, prev);
-  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, rb.members, giant, prev);
+  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, st, round,
+                     act_list, prev);
   uint32_t ent[32];
   const size_t L = round_levels(n, ent, 32);
   for (size_t l = 0; l < L; ++l) {
@@ -1237,25 +1279,27 @@ void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void*
     uint32_t* out_r = rb.proot[l & 1];
     if (l == 0) {
       if (gbytes == 4)
-        hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
-                           rb.members, rb.root, alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r,
-                           rb.samp, d.flags, prev, cnt);
+        hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
+                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
       else
-        hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
-                           rb.members, rb.root, alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r,
-                           rb.samp, d.flags, prev, cnt);
+        hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
+                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
     } else {
       if (gbytes == 4)
-        hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
-                           rb.members, rb.root, alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r,
-                           rb.samp, d.flags, prev, nullptr);
+        hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
+                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
       else
-        hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, level, seg_total, giant,
-                           rb.members, rb.root, alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r,
-                           rb.samp, d.flags, prev, nullptr);
+        hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
+                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
     }
   }
-  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, prev, cur);
+  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
+                     act_list, prev, cur);
+}
+
+void launch_round_start(const RoundBufs& rb, hipStream_t s) {
+  launch_fill_u32(rb.totals, 0, kStWords, s);
+  launch_fill_u32(rb.totals + kStGiantRound, kNoRoot, 2, s);  // no giant round root, none elected
 }
 
 void launch_sum_all(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out_a, void* out_g,

@@ -77,15 +77,15 @@ void launch_dsu_reset(const DsuDev& d, hipStream_t s);
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
                           uint64_t* samp, hipStream_t s);
 struct RoundBufs {
-  uint32_t* root;      // n      root of every vertex this round
-  uint32_t* cnt;       // n      members per root (zero between rounds)
-  uint32_t* off;       // n      segment start per root
-  uint32_t* cursor;    // n      scatter cursor per root
-  uint32_t* members;   // n      vertices grouped by root
-  uint32_t* totals;    // [0] segment allocation cursor (= level-1 entries), [1] giant root or ~0
-  uint64_t* part[2];   // partial-XOR entries of the multi-level reduce, 2*ceil(n/64) entries of 128 u64
-  uint32_t* proot[2];  // root of each partial entry
-  uint64_t* samp;      // n      sample record per supernode root
+  uint32_t* root;        // n      root of every walked vertex this round
+  uint32_t* cnt;         // n      members per root (zero between rounds)
+  uint32_t* off;         // n      segment start per root
+  uint32_t* cursor;      // n      scatter cursor per root
+  uint32_t* members[2];  // n each vertices grouped by root, by round parity (also the next active list)
+  uint32_t* totals;      // 8      round state words (entry counts, giant roots, walk mode)
+  uint64_t* part[2];     // partial-XOR entries of the multi-level reduce, 2*ceil(n/64)+2 entries of 128 u64
+  uint32_t* proot[2];    // root of each partial entry
+  uint64_t* samp;        // n      sample record per supernode root
 };
 // Number of reduce levels for n vertices (entries_per_level gets a bound on each level's entries).
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
@@ -94,6 +94,8 @@ size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
 // the vertices may be summed from its complement.
 void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s);
+// Reset the round state of a query (before round 1).
+void launch_round_start(const RoundBufs& rb, hipStream_t s);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
 void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,
                           hipStream_t s);

@@ -223,14 +223,15 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
 
   // Borůvka round buffers
   const uint32_t waves1 = (n + 63) / 64;
-  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_members.alloc(n); r_samp.alloc(n);
-  r_totals.alloc(4);
+  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
+  r_members[0].alloc(n); r_members[1].alloc(n);
+  r_totals.alloc(8);
   for (int i = 0; i < 2; ++i) {
     r_part[i].alloc((size_t)(2 * waves1 + 2) * 128);
     r_proot[i].alloc(2 * waves1 + 2);
   }
-  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, r_members.p, r_totals.p, {r_part[0].p, r_part[1].p},
-                 {r_proot[0].p, r_proot[1].p}, r_samp.p};
+  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
+                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
@@ -607,6 +608,7 @@ void GzAlg::boruvka() {
     const bool giant_ok = sketches_sum_to_zero();
     launch_dsu_reset(dsu, st);
     launch_fill_u32(r_cnt.p, 0, n, st);
+    launch_round_start(rb, st);
     if (prof) prof_begin(1, 0, false, 0);
     launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, r_samp.p, st);
     GZ_HIP(hipGetLastError());
