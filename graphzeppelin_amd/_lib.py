@@ -158,6 +158,7 @@ _SIGS = {
     "gz_alg_footprint_bytes": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.POINTER(_AlgConfig)]),
     "gz_alg_digest": (ctypes.c_int, [_vp, _u64p]),
     "gz_alg_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_alg_set_query_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_alg_get_profile": (ctypes.c_int, [_vp, ctypes.POINTER(AlgProfile)]),
     # include/gz_serving.h
     "gz_alg_serve_trace": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _sz, _vp, _vp, ctypes.POINTER(ServeStats)]),
@@ -527,6 +528,10 @@ class CCSketchAlg:
 
     def set_profiling(self, enable: bool = True):
         _check(lib().gz_alg_set_profiling(self._h, 1 if enable else 0))
+
+    def set_query_graphs(self, enable: bool = True):
+        """Replay each query's first Borůvka rounds from a HIP graph (default) or launch them one by one."""
+        _check(lib().gz_alg_set_query_graphs(self._h, 1 if enable else 0))
 
     def get_profile(self) -> dict:
         p = AlgProfile()

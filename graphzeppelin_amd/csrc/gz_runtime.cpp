@@ -269,6 +269,8 @@ GzAlg::~GzAlg() {
   for (auto& s : stages)
     if (s.ev) (void)hipEventDestroy(s.ev);
   if (eager_ev) (void)hipEventDestroy(eager_ev);
+  for (auto& g : query_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -598,29 +600,54 @@ bool GzAlg::sketches_sum_to_zero() {
   return zero_sum == 1;
 }
 
-// boruvka_emulation, src/cc_sketch_alg.cpp:464-513.  Rounds are queued in groups; every kernel
-// of round r is a no-op once round r-1 changed nothing, so the host reads the per-round flags once
-// per group instead of once per round.
+// DSU reset, round 0 and rounds 1..last, as one stream sequence (captured into a graph or launched
+// directly).  Fills are kernels, so the sequence holds no memset nodes.
+void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
+  launch_dsu_reset(dsu, st);
+  launch_fill_u32(r_cnt.p, 0, n, st);
+  launch_round_start(rb, st);
+  launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, r_samp.p, st);
+  for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
+}
+
+hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
+  hipGraphExec_t& slot = query_graph[giant_ok ? 1 : 0];
+  if (slot) return slot;
+  GZ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  enqueue_query_prefix(giant_ok, last);
+  hipGraph_t graph = nullptr;
+  const hipError_t cap = hipStreamEndCapture(st, &graph);
+  if (cap != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    throw GzError(GZ_ERR_DEVICE, std::string("query graph capture: ") + hipGetErrorString(cap));
+  }
+  const hipError_t inst = hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (inst != hipSuccess) {
+    slot = nullptr;
+    throw GzError(GZ_ERR_DEVICE, std::string("query graph instantiate: ") + hipGetErrorString(inst));
+  }
+  return slot;
+}
+
+// boruvka_emulation, src/cc_sketch_alg.cpp:464-513.  Every kernel of round r is a no-op once
+// round r-1 changed nothing, so rounds are queued ahead and the host reads the per-round flags
+// once per group: first the graph prefix (round 0 and rounds 1..K), then groups of 4.
 void GzAlg::boruvka() {
   update_locked = true;
   try {
     const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
     const bool giant_ok = sketches_sum_to_zero();
-    launch_dsu_reset(dsu, st);
-    launch_fill_u32(r_cnt.p, 0, n, st);
-    launch_round_start(rb, st);
-    if (prof) prof_begin(1, 0, false, 0);
-    launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, r_samp.p, st);
-    GZ_HIP(hipGetLastError());
     const uint32_t last_round = geo.samples - 1;  // round index `samples` would be out of samples
-    uint32_t next = 1, group = 2;
+    const uint32_t first = std::min(kGraphRounds, last_round);
+    if (prof) prof_begin(1, 0, false, 0);
+    if (use_graphs) GZ_HIP(hipGraphLaunch(get_query_graph(giant_ok, first), st));
+    else enqueue_query_prefix(giant_ok, first);
+    GZ_HIP(hipGetLastError());
+    uint32_t next = first + 1;
     int64_t rounds = -1;
     while (true) {
-      const uint32_t hi = std::min(next + group - 1, last_round);
-      for (uint32_t r = next; r <= hi && next <= last_round; ++r)
-        launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
-      GZ_HIP(hipGetLastError());
-      const uint32_t upto = std::max(hi, next - 1);
+      const uint32_t upto = next - 1;
       GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)(upto + 1) * 4, hipMemcpyDeviceToHost, st));
       GZ_HIP(hipStreamSynchronize(st));
       for (uint32_t r = 0; r <= upto; ++r)
@@ -631,8 +658,10 @@ void GzAlg::boruvka() {
       if (rounds >= 0) break;
       if (upto >= last_round)
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
-      next = upto + 1;
-      group = 4;
+      const uint32_t hi = std::min(next + 3, last_round);
+      for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
+      GZ_HIP(hipGetLastError());
+      next = hi + 1;
     }
     if (prof) {
       prof_end();
@@ -1100,6 +1129,12 @@ int gz_alg_set_profiling(gz_alg* a, int enable) {
     }
     A.prof = enable != 0;
     A.profile = gz_alg_profile{};
+  });
+}
+int gz_alg_set_query_graphs(gz_alg* a, int enable) {
+  return guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    a->a->use_graphs = enable != 0;
   });
 }
 int gz_alg_get_profile(gz_alg* a, gz_alg_profile* out) {

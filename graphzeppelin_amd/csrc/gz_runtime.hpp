@@ -244,6 +244,17 @@ class GzAlg {
   void ensure_answer();
   void boruvka();
   uint32_t read_flag(int slot);
+
+  // Query prefix (DSU reset, round 0 and rounds 1..K) replayed from a HIP graph: every parameter is
+  // fixed per instance (round state lives on the device), so one graph launch replaces ~9 kernel
+  // launches per round.  One graph per giant_ok value, built on first use.
+ public:
+  bool use_graphs = true;
+  static constexpr uint32_t kGraphRounds = 12;
+ private:
+  hipGraphExec_t query_graph[2] = {nullptr, nullptr};
+  void enqueue_query_prefix(bool giant_ok, uint32_t last);
+  hipGraphExec_t get_query_graph(bool giant_ok, uint32_t last);
 };
 
 // Runs f, converting exceptions to a gz_status and the thread's last-error message.

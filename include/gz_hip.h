@@ -167,6 +167,10 @@ typedef struct {
 } gz_alg_profile;
 int gz_alg_set_profiling(gz_alg* a, int enable);
 int gz_alg_get_profile(gz_alg* a, gz_alg_profile* out);
+/* No counterpart in the reference (a device execution knob).  enable != 0 (the default): a query
+ * replays DSU reset, round 0 and the first speculative Borůvka rounds from a HIP graph built on
+ * the instance's first query; 0: the same kernels are launched one by one.  Results are identical. */
+int gz_alg_set_query_graphs(gz_alg* a, int enable);
 /* Device footprint in bytes of an instance with these parameters (sketch slab + work buffers). */
 uint64_t gz_alg_footprint_bytes(uint32_t num_vertices, const gz_alg_config* cfg);
 

@@ -310,3 +310,23 @@ def test_one_sided_batches_and_raw_deltas_query_like_the_oracle():
     rparents, rc = ref.connected_components()
     assert alg.last_query_rounds == ref.last_query_rounds
     assert np.array_equal(cc.canonical_labels(), oracle.canonical(rparents))
+
+
+def test_query_graph_replays_equal_direct_launches():
+    # the query prefix is replayed from a HIP graph; eager ingest and synchronisations run between
+    # replays, and every replay must equal the same kernels launched one by one
+    pairs, _ = streams.dynamic_erdos_stream(900, 0.003, 0.3, 0.2, 4, 8)
+    g = gz.CCSketchAlg(900, 19, gz.CCAlgConfiguration(eager_dsu=False))
+    d = gz.CCSketchAlg(900, 19, gz.CCAlgConfiguration(eager_dsu=False))
+    d.set_query_graphs(False)
+    step = pairs.shape[0] // 6 + 1
+    for i in range(0, pairs.shape[0], step):
+        chunk = pairs[i:i + step]
+        g.ingest_pairs(chunk)
+        d.ingest_pairs(chunk)
+        cg, cd = g.connected_components(), d.connected_components()
+        assert g.last_query_rounds == d.last_query_rounds
+        assert np.array_equal(cg.canonical_labels(), cd.canonical_labels())
+        c, lab = oracle.exact_cc(900, pairs[:i + step])
+        assert cg.size() == c and np.array_equal(cg.canonical_labels(), lab)
+        assert g.point_query(0, 1) == d.point_query(0, 1)
