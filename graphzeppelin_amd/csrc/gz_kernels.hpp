@@ -73,9 +73,6 @@ void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const Ds
 
 // ---- Borůvka ----
 void launch_dsu_reset(const DsuDev& d, hipStream_t s);
-// samp: n u64 sample records (the same buffer as RoundBufs::samp)
-void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
-                          uint64_t* samp, hipStream_t s);
 struct RoundBufs {
   uint32_t* root;        // n      root of every walked vertex this round
   uint32_t* cnt;         // n      members per root (zero between rounds)
@@ -86,7 +83,12 @@ struct RoundBufs {
   uint64_t* part[2];     // partial-XOR entries of the multi-level reduce, 2*ceil(n/64)+2 entries of 128 u64
   uint32_t* proot[2];    // root of each partial entry
   uint64_t* samp;        // n      sample record per supernode root
+  uint64_t* det_a;       // n      deterministic bucket of every vertex (alpha), copied in round 0
+  void* det_g;           // n      its checksum (gamma width)
 };
+// Writes rb.samp (a sample record per vertex) and the compact deterministic buckets rb.det_a/det_g.
+void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
+                          const RoundBufs& rb, hipStream_t s);
 // Number of reduce levels for n vertices (entries_per_level gets a bound on each level's entries).
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
 // Queue round `round` (>= 1); every kernel of it is a no-op when d.mod[round - 1] == 0.

@@ -171,7 +171,10 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint32_t t = c.tile ? c.tile : 1024;
   const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
-  const uint64_t rounds = (uint64_t)n * 40 + (uint64_t)n * 16 /*sf hash*/;
+  // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
+  // forest edge, compact deterministic bucket (8 + gamma), partial entries (2 x 1 KiB per 32
+  // vertices), spanning-forest hash set (16)
+  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 8 + gbytes + 64 + 16) + 4096;
   return slab + ingest + rounds;
 }
 
@@ -224,6 +227,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   // Borůvka round buffers
   const uint32_t waves1 = (n + 63) / 64;
   r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
+  r_det_a.alloc(n); r_det_g.alloc((size_t)n * gb);
   r_members[0].alloc(n); r_members[1].alloc(n);
   r_totals.alloc(8);
   for (int i = 0; i < 2; ++i) {
@@ -231,7 +235,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
     r_proot[i].alloc(2 * waves1 + 2);
   }
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
-                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p};
+                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p, r_det_a.p, r_det_g.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
@@ -606,7 +610,7 @@ void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
   launch_dsu_reset(dsu, st);
   launch_fill_u32(r_cnt.p, 0, n, st);
   launch_round_start(rb, st);
-  launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, r_samp.p, st);
+  launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, rb, st);
   for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
 }
 

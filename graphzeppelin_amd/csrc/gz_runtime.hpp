@@ -142,7 +142,8 @@ class GzAlg {
   DsuDev dsu{};
   // Borůvka round buffers
   DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_totals, mod;
-  DevBuf<uint64_t> r_part[2], r_samp;
+  DevBuf<uint64_t> r_part[2], r_samp, r_det_a;
+  DevBuf<uint8_t> r_det_g;
   DevBuf<uint32_t> r_proot[2];
   RoundBufs rb{};
   // device stream ingest (pending pairs -> grouping -> tiles)
