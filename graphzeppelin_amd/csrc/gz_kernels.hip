@@ -565,7 +565,7 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
 template <class G>
 __global__ void __launch_bounds__(256)
 k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint64_t* __restrict__ samp,
-                uint32_t* __restrict__ flags, uint64_t* __restrict__ det_a_out, G* __restrict__ det_g_out) {
+                uint32_t* __restrict__ flags) {
   constexpr uint32_t V = 4;  // vertices per wave iteration: all their loads are issued first
   const uint32_t lane = lane_id();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
@@ -594,11 +594,7 @@ k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma,
       if (v0 + k >= g.n) break;
       uint64_t idx;
       const int code = wave_sample<G>(det_a[k], det_g[k], a[k], gm[k], valid, g.chk_bf, &idx);
-      if (lane == 0) {
-        samp[v0 + k] = sample_record(code, idx, g.n, flags);
-        det_a_out[v0 + k] = det_a[k];  // the deterministic bucket does not depend on the round
-        det_g_out[v0 + k] = det_g[k];
-      }
+      if (lane == 0) samp[v0 + k] = sample_record(code, idx, g.n, flags);
     }
   }
 }
@@ -844,8 +840,7 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
             const uint32_t* __restrict__ vroot, const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g,
             uint32_t round, const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
             uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
-            uint32_t* __restrict__ flags, const uint64_t* __restrict__ det_a, const G* __restrict__ det_g,
-            const uint32_t* __restrict__ prev_mod) {
+            uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
   const uint32_t lane = lane_id();
   const uint32_t giant = st[kStGiantRound];
@@ -919,7 +914,7 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
 
   uint64_t acc_a = 0, acc_g = 0, all_a = 0, all_g = 0;
   uint32_t cur = first_root, seg_start = p0;
-  constexpr uint32_t U = 16;  // loads in flight per lane
+  constexpr uint32_t U = FIRST ? 8 : 16;  // loads in flight per lane
   for (uint32_t j = 0; j < len; j += U) {
     uint64_t va[U], vg[U];
 #pragma unroll
@@ -928,15 +923,9 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
       vg[k] = 0;
       if (j + k < len && loads) {
         if (FIRST) {
-          const uint32_t mv = rd_lane_u32(m, j + k);
-          if (in_col) {
-            const size_t o = (size_t)mv * g.stride + boff;
-            va[k] = alpha[o];
-            vg[k] = (uint64_t)gamma[o];
-          } else {  // lane D: the deterministic bucket, from the compact copy made in round 0
-            va[k] = det_a[mv];
-            vg[k] = (uint64_t)det_g[mv];
-          }
+          const size_t o = (size_t)rd_lane_u32(m, j + k) * g.stride + boff;
+          va[k] = alpha[o];
+          vg[k] = (uint64_t)gamma[o];
         } else if (rd_lane_u32(rt, j + k) != kNoRoot) {  // empty partial slots hold no value
           const uint64_t* e = in_part + (size_t)(p0 + j + k) * 2 * kWave;
           va[k] = e[lane];
@@ -1228,11 +1217,9 @@ void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, 
   uint64_t* samp = rb.samp;
   const uint32_t grid = grid_for((size_t)(g.n + 3) / 4 * kWave, 256, 16384);
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, samp, d.flags, rb.det_a,
-                       (uint32_t*)rb.det_g);
+    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, samp, d.flags);
   else
-    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, samp, d.flags, rb.det_a,
-                       (uint64_t*)rb.det_g);
+    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, samp, d.flags);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr, 0u,
                      nullptr, nullptr, d.mod);
 }
@@ -1285,21 +1272,17 @@ void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void*
     if (l == 0) {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, rb.det_a,
-                           (const uint32_t*)rb.det_g, prev);
+                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, rb.det_a,
-                           (const uint64_t*)rb.det_g, prev);
+                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
     } else {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, rb.det_a,
-                           (const uint32_t*)rb.det_g, prev);
+                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, rb.det_a,
-                           (const uint64_t*)rb.det_g, prev);
+                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
     }
   }
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,

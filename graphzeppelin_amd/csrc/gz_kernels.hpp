@@ -83,10 +83,8 @@ struct RoundBufs {
   uint64_t* part[2];     // partial-XOR entries of the multi-level reduce, 2*ceil(n/64)+2 entries of 128 u64
   uint32_t* proot[2];    // root of each partial entry
   uint64_t* samp;        // n      sample record per supernode root
-  uint64_t* det_a;       // n      deterministic bucket of every vertex (alpha), copied in round 0
-  void* det_g;           // n      its checksum (gamma width)
 };
-// Writes rb.samp (a sample record per vertex) and the compact deterministic buckets rb.det_a/det_g.
+// Writes rb.samp: a sample record per vertex.
 void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
                           const RoundBufs& rb, hipStream_t s);
 // Number of reduce levels for n vertices (entries_per_level gets a bound on each level's entries).

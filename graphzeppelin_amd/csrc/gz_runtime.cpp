@@ -172,9 +172,8 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
   // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
-  // forest edge, compact deterministic bucket (8 + gamma), partial entries (2 x 1 KiB per 32
-  // vertices), spanning-forest hash set (16)
-  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 8 + gbytes + 64 + 16) + 4096;
+  // forest edge (8 each), partial entries (2 x 1 KiB per 32 vertices), spanning-forest hash set (16)
+  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 64 + 16) + 4096;
   return slab + ingest + rounds;
 }
 
@@ -227,7 +226,6 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   // Borůvka round buffers
   const uint32_t waves1 = (n + 63) / 64;
   r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
-  r_det_a.alloc(n); r_det_g.alloc((size_t)n * gb);
   r_members[0].alloc(n); r_members[1].alloc(n);
   r_totals.alloc(8);
   for (int i = 0; i < 2; ++i) {
@@ -235,7 +233,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
     r_proot[i].alloc(2 * waves1 + 2);
   }
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
-                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p, r_det_a.p, r_det_g.p};
+                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
@@ -273,8 +271,7 @@ GzAlg::~GzAlg() {
   for (auto& s : stages)
     if (s.ev) (void)hipEventDestroy(s.ev);
   if (eager_ev) (void)hipEventDestroy(eager_ev);
-  for (auto& g : query_graph)
-    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& kv : query_graphs) (void)hipGraphExecDestroy(kv.second);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -615,7 +612,7 @@ void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
 }
 
 hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
-  hipGraphExec_t& slot = query_graph[giant_ok ? 1 : 0];
+  hipGraphExec_t& slot = query_graphs[last * 2 + (giant_ok ? 1 : 0)];
   if (slot) return slot;
   GZ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   enqueue_query_prefix(giant_ok, last);
@@ -623,12 +620,13 @@ hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
   const hipError_t cap = hipStreamEndCapture(st, &graph);
   if (cap != hipSuccess || !graph) {
     if (graph) (void)hipGraphDestroy(graph);
+    query_graphs.erase(last * 2 + (giant_ok ? 1 : 0));
     throw GzError(GZ_ERR_DEVICE, std::string("query graph capture: ") + hipGetErrorString(cap));
   }
   const hipError_t inst = hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (inst != hipSuccess) {
-    slot = nullptr;
+    query_graphs.erase(last * 2 + (giant_ok ? 1 : 0));
     throw GzError(GZ_ERR_DEVICE, std::string("query graph instantiate: ") + hipGetErrorString(inst));
   }
   return slot;
@@ -636,14 +634,15 @@ hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
 
 // boruvka_emulation, src/cc_sketch_alg.cpp:464-513.  Every kernel of round r is a no-op once
 // round r-1 changed nothing, so rounds are queued ahead and the host reads the per-round flags
-// once per group: first the graph prefix (round 0 and rounds 1..K), then groups of 4.
+// once per group: first the graph prefix (round 0 and rounds 1..K), then groups of 2.
 void GzAlg::boruvka() {
   update_locked = true;
   try {
     const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
     const bool giant_ok = sketches_sum_to_zero();
     const uint32_t last_round = geo.samples - 1;  // round index `samples` would be out of samples
-    const uint32_t first = std::min(kGraphRounds, last_round);
+    const uint32_t want = last_rounds ? (uint32_t)last_rounds + 1 : kFirstGraphRounds;
+    const uint32_t first = std::max(1u, std::min(want, last_round));
     if (prof) prof_begin(1, 0, false, 0);
     if (use_graphs) GZ_HIP(hipGraphLaunch(get_query_graph(giant_ok, first), st));
     else enqueue_query_prefix(giant_ok, first);
@@ -662,7 +661,7 @@ void GzAlg::boruvka() {
       if (rounds >= 0) break;
       if (upto >= last_round)
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
-      const uint32_t hi = std::min(next + 3, last_round);
+      const uint32_t hi = std::min(next + 1, last_round);
       for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
       GZ_HIP(hipGetLastError());
       next = hi + 1;

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -142,8 +143,7 @@ class GzAlg {
   DsuDev dsu{};
   // Borůvka round buffers
   DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_totals, mod;
-  DevBuf<uint64_t> r_part[2], r_samp, r_det_a;
-  DevBuf<uint8_t> r_det_g;
+  DevBuf<uint64_t> r_part[2], r_samp;
   DevBuf<uint32_t> r_proot[2];
   RoundBufs rb{};
   // device stream ingest (pending pairs -> grouping -> tiles)
@@ -248,12 +248,14 @@ class GzAlg {
 
   // Query prefix (DSU reset, round 0 and rounds 1..K) replayed from a HIP graph: every parameter is
   // fixed per instance (round state lives on the device), so one graph launch replaces ~9 kernel
-  // launches per round.  One graph per giant_ok value, built on first use.
+  // launches per round.  K follows the previous query's round count plus one (queries on one
+  // stream need similar counts), so few no-op rounds are dispatched; graphs are built on first use
+  // and kept per (K, giant_ok).
  public:
   bool use_graphs = true;
-  static constexpr uint32_t kGraphRounds = 12;
+  static constexpr uint32_t kFirstGraphRounds = 12;
  private:
-  hipGraphExec_t query_graph[2] = {nullptr, nullptr};
+  std::map<uint32_t, hipGraphExec_t> query_graphs;
   void enqueue_query_prefix(bool giant_ok, uint32_t last);
   hipGraphExec_t get_query_graph(bool giant_ok, uint32_t last);
 };
