@@ -68,7 +68,7 @@ struct Geo {
   uint32_t cps;      // cols_per_sample
   uint32_t samples;  // num_samples
   uint32_t nb;       // buckets per sketch = C*D + 1 (deterministic bucket last)
-  uint32_t stride;   // elements between consecutive sketches in the device planes
+  uint32_t row;      // bytes between consecutive sketches in the slab (nb records, 256-B aligned rows)
   uint32_t Dp;       // LDS column pitch (D rounded up to 1 mod 16 to spread LDS banks)
   uint64_t seed;     // sketch seed (checksum seed)
   uint64_t chk_bf;   // bitflip of the checksum seed

@@ -68,6 +68,55 @@ __device__ inline void st_rlx(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---- the slab: packed bucket records -----------------------------------------------------------
+// Bucket b of sketch v is the record {u64 alpha; G gamma} -- the reference's packed Bucket, 12 or 16
+// bytes -- at byte v * g.row + b * (8 + sizeof(G)).  The low depths of a column, which most
+// updates touch, are then adjacent in one cache line with their checksums.  A 12-byte record is
+// only 4-byte aligned, so it is moved as 32-bit words, and a 64-bit XOR atomic on it becomes two
+// 32-bit ones (XOR acts bit by bit, so the result is the same).
+template <class G>
+__device__ inline size_t rec_off(const Geo& g, uint32_t v, uint32_t b) {
+  return (size_t)v * g.row + (size_t)b * (8 + sizeof(G));
+}
+template <class G>
+__device__ inline void ld_rec(const unsigned char* p, uint64_t* a, G* gm) {
+  if (sizeof(G) == 4) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+    *a = ((uint64_t)w[1] << 32) | w[0];
+    *gm = (G)w[2];
+  } else {
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(p);
+    *a = w[0];
+    *gm = (G)w[1];
+  }
+}
+template <class G>
+__device__ inline void st_rec(unsigned char* p, uint64_t a, G gm) {
+  if (sizeof(G) == 4) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(p);
+    w[0] = (uint32_t)a;
+    w[1] = (uint32_t)(a >> 32);
+    w[2] = (uint32_t)gm;
+  } else {
+    uint64_t* w = reinterpret_cast<uint64_t*>(p);
+    w[0] = a;
+    w[1] = (uint64_t)gm;
+  }
+}
+template <class G>
+__device__ inline void xor_rec_atomic(unsigned char* p, uint64_t a, G gm) {
+  if (sizeof(G) == 4) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(p);
+    if ((uint32_t)a) gbl_xor(w, (uint32_t)a);
+    if ((uint32_t)(a >> 32)) gbl_xor(w + 1, (uint32_t)(a >> 32));
+    if (gm) gbl_xor(w + 2, (uint32_t)gm);
+  } else {
+    uint64_t* w = reinterpret_cast<uint64_t*>(p);
+    if (a) gbl_xor(w, a);
+    if (gm) gbl_xor(w + 1, (uint64_t)gm);
+  }
+}
+
 // ---- DSU (include/dsu.h:101-174 restated for the device) --------------------------------------
 // Priorities are a bijective 32-bit mix of the vertex id, so (priority, id) is a strict total
 // order and linking the lower-priority root under the higher one can never form a cycle.
@@ -218,7 +267,7 @@ template <class G, bool RAW, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntiles_dev, uint32_t ntiles_host,
                uint32_t* __restrict__ work_counter, const uint32_t* __restrict__ dsts,
-               const uint64_t* __restrict__ raw_idx, uint64_t* __restrict__ alpha, G* __restrict__ gamma, Geo g,
+               const uint64_t* __restrict__ raw_idx, unsigned char* __restrict__ slab, Geo g,
                const uint64_t* __restrict__ col_bf) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t C = g.C, D = g.D, Dp = g.Dp;
@@ -289,8 +338,9 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     // next tile, read after the barrier below
     if (threadIdx.x == 0) s_tile = t + gridDim.x;
 
-    // write-back: read-xor-write of the vertex rows, four buckets per thread in flight
-    const size_t base = (size_t)tl.vertex * g.stride;
+    // write-back: read-xor-write of the vertex's records, four buckets per thread in flight
+    unsigned char* row = slab + (size_t)tl.vertex * g.row;
+    constexpr uint32_t RB = 8 + sizeof(G);
     for (uint32_t b0 = 0; b0 < nb; b0 += 4 * BLOCK) {
       uint64_t da[4], ga[4];
       G dg[4], gg[4];
@@ -319,27 +369,18 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
       }
       if (tl.split) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if (da[k]) gbl_xor(alpha + base + bi[k], da[k]);
-          if (dg[k]) gbl_xor(gamma + base + bi[k], dg[k]);
-        }
+        for (int k = 0; k < 4; ++k)
+          if ((da[k] | (uint64_t)dg[k]) != 0) xor_rec_atomic<G>(row + (size_t)bi[k] * RB, da[k], dg[k]);
       } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           ga[k] = 0;
           gg[k] = 0;
-          if ((da[k] | (uint64_t)dg[k]) != 0) {
-            ga[k] = alpha[base + bi[k]];
-            gg[k] = gamma[base + bi[k]];
-          }
+          if ((da[k] | (uint64_t)dg[k]) != 0) ld_rec<G>(row + (size_t)bi[k] * RB, &ga[k], &gg[k]);
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if ((da[k] | (uint64_t)dg[k]) != 0) {
-            alpha[base + bi[k]] = ga[k] ^ da[k];
-            gamma[base + bi[k]] = gg[k] ^ dg[k];
-          }
-        }
+        for (int k = 0; k < 4; ++k)
+          if ((da[k] | (uint64_t)dg[k]) != 0) st_rec<G>(row + (size_t)bi[k] * RB, ga[k] ^ da[k], (G)(gg[k] ^ dg[k]));
       }
     }
     __syncthreads();
@@ -564,7 +605,7 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
 // per vertex; lane l checks depth l, the first good depth is the lowest set bit of the ballot.
 template <class G>
 __global__ void __launch_bounds__(256)
-k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint64_t* __restrict__ samp,
+k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restrict__ samp,
                 uint32_t* __restrict__ flags) {
   constexpr uint32_t V = 4;  // vertices per wave iteration: all their loads are issued first
   const uint32_t lane = lane_id();
@@ -580,13 +621,8 @@ k_sample_round0(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma,
       a[k] = 0;
       gm[k] = 0;
       if (v0 + k < g.n) {
-        const size_t base = (size_t)(v0 + k) * g.stride;
-        det_a[k] = alpha[base + g.nb - 1];
-        det_g[k] = gamma[base + g.nb - 1];
-        if (valid) {
-          a[k] = alpha[base + lane];
-          gm[k] = gamma[base + lane];
-        }
+        ld_rec<G>(slab + rec_off<G>(g, v0 + k, g.nb - 1), &det_a[k], &det_g[k]);
+        if (valid) ld_rec<G>(slab + rec_off<G>(g, v0 + k, lane), &a[k], &gm[k]);
       }
     }
 #pragma unroll
@@ -837,7 +873,7 @@ __device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, ui
 template <class G, bool FIRST>
 __global__ void __launch_bounds__(256)
 k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
-            const uint32_t* __restrict__ vroot, const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g,
+            const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g,
             uint32_t round, const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
             uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
             uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod) {
@@ -923,9 +959,9 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
       vg[k] = 0;
       if (j + k < len && loads) {
         if (FIRST) {
-          const size_t o = (size_t)rd_lane_u32(m, j + k) * g.stride + boff;
-          va[k] = alpha[o];
-          vg[k] = (uint64_t)gamma[o];
+          G gw;
+          ld_rec<G>(slab + rec_off<G>(g, rd_lane_u32(m, j + k), boff), &va[k], &gw);
+          vg[k] = (uint64_t)gw;
         } else if (rd_lane_u32(rt, j + k) != kNoRoot) {  // empty partial slots hold no value
           const uint64_t* e = in_part + (size_t)(p0 + j + k) * 2 * kWave;
           va[k] = e[lane];
@@ -997,11 +1033,12 @@ __global__ void k_point_queries(uint32_t* __restrict__ parent, const uint32_t* _
 // Sketch::sample for any cols_per_sample: deterministic bucket first, then the sample's columns
 // in bucket order; one wave walks them 64 buckets at a time.
 template <class G>
-__global__ void k_sketch_sample(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g,
-                                uint32_t sample_idx, uint32_t* __restrict__ out) {
+__global__ void k_sketch_sample(const unsigned char* __restrict__ slab, Geo g, uint32_t sample_idx,
+                                uint32_t* __restrict__ out) {
   const uint32_t lane = lane_id();
-  const uint64_t det_a = alpha[g.nb - 1];
-  const G det_g = gamma[g.nb - 1];
+  uint64_t det_a;
+  G det_g;
+  ld_rec<G>(slab + rec_off<G>(g, 0, g.nb - 1), &det_a, &det_g);
   int code;
   uint64_t idx = 0;
   if (det_a == 0 && det_g == 0) {
@@ -1014,8 +1051,9 @@ __global__ void k_sketch_sample(const uint64_t* __restrict__ alpha, const G* __r
     const uint32_t b0 = sample_idx * g.cps * g.D, b1 = b0 + g.cps * g.D;
     for (uint32_t b = b0; b < b1; b += kWave) {
       const bool valid = b + lane < b1;
-      const uint64_t a = valid ? alpha[b + lane] : 0;
-      const G gm = valid ? gamma[b + lane] : 0;
+      uint64_t a = 0;
+      G gm = 0;
+      if (valid) ld_rec<G>(slab + rec_off<G>(g, 0, b + lane), &a, &gm);
       const bool good = valid && (G)xxh3_mix(xxh3_prep(a), g.chk_bf) == gm;
       const uint64_t mask = __ballot(good);
       if (mask) {
@@ -1033,78 +1071,51 @@ __global__ void k_sketch_sample(const uint64_t* __restrict__ alpha, const G* __r
 }
 
 template <class G>
-__global__ void k_xor_range(uint64_t* __restrict__ da, G* __restrict__ dg, const uint64_t* __restrict__ sa,
-                            const G* __restrict__ sg, Geo g, uint32_t nsk, uint32_t b0, uint32_t cnt, int with_det) {
+__global__ void k_xor_range(unsigned char* __restrict__ dst, const unsigned char* __restrict__ src, Geo g,
+                            uint32_t nsk, uint32_t b0, uint32_t cnt, int with_det) {
   const uint32_t per = cnt + (with_det ? 1 : 0);
   const size_t total = (size_t)nsk * per;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
     const uint32_t s = (uint32_t)(i / per), k = (uint32_t)(i % per);
     const uint32_t b = k < cnt ? b0 + k : g.nb - 1;
-    const size_t o = (size_t)s * g.stride + b;
-    da[o] ^= sa[o];
-    dg[o] ^= sg[o];
+    const size_t o = rec_off<G>(g, s, b);
+    uint64_t da, sa;
+    G dg, sg;
+    ld_rec<G>(dst + o, &da, &dg);
+    ld_rec<G>(src + o, &sa, &sg);
+    st_rec<G>(dst + o, da ^ sa, (G)(dg ^ sg));
   }
 }
 
-// Packed reference image: record i of sketch v holds {u64 alpha; G gamma} at byte
-// (v * nb + i) * (8 + sizeof(G)).  A 12-byte record is only 4-byte aligned, so it is moved as
-// three 32-bit words.
-template <class G>
-__device__ inline void load_packed(const unsigned char* p, uint64_t* a, G* gm) {
-  if (sizeof(G) == 4) {
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
-    *a = ((uint64_t)w[1] << 32) | w[0];
-    *gm = (G)w[2];
-  } else {
-    const uint64_t* w = reinterpret_cast<const uint64_t*>(p);
-    *a = w[0];
-    *gm = (G)w[1];
-  }
-}
-template <class G>
-__device__ inline void store_packed(unsigned char* p, uint64_t a, G gm) {
-  if (sizeof(G) == 4) {
-    uint32_t* w = reinterpret_cast<uint32_t*>(p);
-    w[0] = (uint32_t)a;
-    w[1] = (uint32_t)(a >> 32);
-    w[2] = (uint32_t)gm;
-  } else {
-    uint64_t* w = reinterpret_cast<uint64_t*>(p);
-    w[0] = a;
-    w[1] = (uint64_t)gm;
-  }
-}
-template <class G, int MODE>  // 0 = xor into planes, 1 = overwrite planes, 2 = pack planes
-__global__ void k_packed(uint64_t* __restrict__ alpha, G* __restrict__ gamma, Geo g, uint32_t v0, uint32_t nv,
+// Packed reference image: record i of sketch v at byte (v * nb + i) * (8 + sizeof(G)); the slab row
+// holds the same records followed by padding.
+template <class G, int MODE>  // 0 = xor image into slab, 1 = overwrite slab, 2 = slab to image
+__global__ void k_packed(unsigned char* __restrict__ slab, Geo g, uint32_t v0, uint32_t nv,
                          unsigned char* __restrict__ packed) {
   const size_t rec = 8 + sizeof(G);
   const size_t total = (size_t)nv * g.nb;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
     const uint32_t v = (uint32_t)(i / g.nb), b = (uint32_t)(i % g.nb);
-    const size_t o = (size_t)(v0 + v) * g.stride + b;
+    unsigned char* r = slab + rec_off<G>(g, v0 + v, b);
+    uint64_t a, pa;
+    G gm, pg;
+    ld_rec<G>(r, &a, &gm);
     if (MODE == 2) {
-      store_packed<G>(packed + i * rec, alpha[o], gamma[o]);
+      st_rec<G>(packed + i * rec, a, gm);
     } else {
-      uint64_t a;
-      G gm;
-      load_packed<G>(packed + i * rec, &a, &gm);
-      if (MODE == 0) { alpha[o] ^= a; gamma[o] ^= gm; }
-      else { alpha[o] = a; gamma[o] = gm; }
+      ld_rec<G>(packed + i * rec, &pa, &pg);
+      if (MODE == 0) st_rec<G>(r, a ^ pa, (G)(gm ^ pg));
+      else st_rec<G>(r, pa, pg);
     }
   }
 }
 
-// Order-sensitive digest of every vertex's bucket array:
-//   D = sum over vertices v, buckets b of fmix64(x + pos), pos = v*nb + b,
-//   x = alpha ^ fmix64(gamma ^ pos * 0x9E3779B97F4A7C15)   (all mod 2^64)
-// A sum is independent of reduction order, so the value is deterministic.
 // Bucket-wise XOR of every vertex sketch into out_a / out_g (nb entries each, zero on entry): one
 // thread per (bucket, chunk of kSumChunk vertices), one device-scope atomic per thread.
 constexpr uint32_t kSumChunk = 1024;
 template <class G>
 __global__ void __launch_bounds__(256)
-k_sum_all(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, uint64_t* __restrict__ out_a,
-          G* __restrict__ out_g) {
+k_sum_all(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restrict__ out_a, G* __restrict__ out_g) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= g.nb) return;
   const uint32_t v0 = blockIdx.y * kSumChunk;
@@ -1112,23 +1123,32 @@ k_sum_all(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g
   uint64_t a = 0;
   G gm = 0;
   for (uint32_t v = v0; v < v1; ++v) {
-    a ^= alpha[(size_t)v * g.stride + b];
-    gm ^= gamma[(size_t)v * g.stride + b];
+    uint64_t ra;
+    G rg;
+    ld_rec<G>(slab + rec_off<G>(g, v, b), &ra, &rg);
+    a ^= ra;
+    gm ^= rg;
   }
   if (a) gbl_xor(out_a + b, a);
   if (gm) gbl_xor(out_g + b, gm);
 }
 
+// Order-sensitive digest of every vertex's bucket array:
+//   D = sum over vertices v, buckets b of fmix64(x + pos), pos = v*nb + b,
+//   x = alpha ^ fmix64(gamma ^ pos * 0x9E3779B97F4A7C15)   (all mod 2^64)
+// A sum is independent of reduction order, so the value is deterministic.
 template <class G>
 __global__ void __launch_bounds__(256)
-k_digest(const uint64_t* __restrict__ alpha, const G* __restrict__ gamma, Geo g, unsigned long long* __restrict__ out) {
+k_digest(const unsigned char* __restrict__ slab, Geo g, unsigned long long* __restrict__ out) {
   const size_t total = (size_t)g.n * g.nb;
   uint64_t acc = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
     const uint32_t v = (uint32_t)(i / g.nb), b = (uint32_t)(i % g.nb);
-    const size_t o = (size_t)v * g.stride + b;
+    uint64_t a;
+    G gm;
+    ld_rec<G>(slab + rec_off<G>(g, v, b), &a, &gm);
     const uint64_t pos = i;
-    const uint64_t x = alpha[o] ^ mix64((uint64_t)gamma[o] ^ (pos * 0x9E3779B97F4A7C15ULL));
+    const uint64_t x = a ^ mix64((uint64_t)gm ^ (pos * 0x9E3779B97F4A7C15ULL));
     acc += mix64(x + pos);
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -1159,7 +1179,7 @@ size_t update_tiles_lds_bytes(const Geo& g, int gbytes) {
 
 void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
                          uint32_t grid_cap, uint32_t* work_counter, const uint32_t* dsts, const uint64_t* raw_idx,
-                         uint64_t* alpha, void* gamma, const Geo& g, const uint64_t* col_bf, hipStream_t s) {
+                         unsigned char* slab, const Geo& g, const uint64_t* col_bf, hipStream_t s) {
   constexpr int B = 256;
   (void)work_counter;
   // tile count known on the host: one workgroup per tile (capped, then grid-stride);
@@ -1170,17 +1190,17 @@ void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_d
   if (gbytes == 4) {
     if (raw_idx)
       hipLaunchKernelGGL((k_update_tiles<uint32_t, true, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint32_t*)gamma, g, col_bf);
+                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
     else
       hipLaunchKernelGGL((k_update_tiles<uint32_t, false, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint32_t*)gamma, g, col_bf);
+                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
   } else {
     if (raw_idx)
       hipLaunchKernelGGL((k_update_tiles<uint64_t, true, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint64_t*)gamma, g, col_bf);
+                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
     else
       hipLaunchKernelGGL((k_update_tiles<uint64_t, false, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, alpha, (uint64_t*)gamma, g, col_bf);
+                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
   }
 }
 
@@ -1212,14 +1232,14 @@ void launch_dsu_reset(const DsuDev& d, hipStream_t s) {
                      d.sf_mask + 1, d.sf_edge, d.flags, d.mod, d.nmod);
 }
 
-void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
-                          const RoundBufs& rb, hipStream_t s) {
+void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
+                          hipStream_t s) {
   uint64_t* samp = rb.samp;
   const uint32_t grid = grid_for((size_t)(g.n + 3) / 4 * kWave, 256, 16384);
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, samp, d.flags);
+    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags);
   else
-    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, samp, d.flags);
+    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr, 0u,
                      nullptr, nullptr, d.mod);
 }
@@ -1243,7 +1263,7 @@ size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) 
   return L;
 }
 
-void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
+void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s) {
   const uint32_t n = g.n;
   const uint32_t* prev = d.mod + (round - 1);
@@ -1272,17 +1292,17 @@ void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void*
     if (l == 0) {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
     } else {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint32_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           alpha, (const uint64_t*)gamma, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
     }
   }
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
@@ -1294,28 +1314,25 @@ void launch_round_start(const RoundBufs& rb, hipStream_t s) {
   launch_fill_u32(rb.totals + kStGiantRound, kNoRoot, 2, s);  // no giant round root, none elected
 }
 
-void launch_sum_all(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out_a, void* out_g,
-                    hipStream_t s) {
+void launch_sum_all(int gbytes, const unsigned char* slab, const Geo& g, uint64_t* out_a, void* out_g, hipStream_t s) {
   launch_fill_u64(out_a, 0, g.nb, s);
   if (gbytes == 4) launch_fill_u32((uint32_t*)out_g, 0, g.nb, s);
   else launch_fill_u64((uint64_t*)out_g, 0, g.nb, s);
   const dim3 grid((g.nb + 255) / 256, (g.n + kSumChunk - 1) / kSumChunk);
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_sum_all<uint32_t>, grid, dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, out_a,
-                       (uint32_t*)out_g);
+    hipLaunchKernelGGL(k_sum_all<uint32_t>, grid, dim3(256), 0, s, slab, g, out_a, (uint32_t*)out_g);
   else
-    hipLaunchKernelGGL(k_sum_all<uint64_t>, grid, dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, out_a,
-                       (uint64_t*)out_g);
+    hipLaunchKernelGGL(k_sum_all<uint64_t>, grid, dim3(256), 0, s, slab, g, out_a, (uint64_t*)out_g);
 }
 
-void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s) {
+void launch_digest(int gbytes, const unsigned char* slab, const Geo& g, uint64_t* out, hipStream_t s) {
   launch_fill_u64(out, 0, 1, s);
   const uint32_t grid = grid_for((size_t)g.n * g.nb, 256, 4096);
   auto* o = reinterpret_cast<unsigned long long*>(out);
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_digest<uint32_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint32_t*)gamma, g, o);
+    hipLaunchKernelGGL(k_digest<uint32_t>, dim3(grid), dim3(256), 0, s, slab, g, o);
   else
-    hipLaunchKernelGGL(k_digest<uint64_t>, dim3(grid), dim3(256), 0, s, alpha, (const uint64_t*)gamma, g, o);
+    hipLaunchKernelGGL(k_digest<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, o);
 }
 
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s) {
@@ -1328,51 +1345,47 @@ void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* q
   hipLaunchKernelGGL(k_point_queries, dim3(grid_for(nq, 256)), dim3(256), 0, s, d.parent, qa, qb, nq, out);
 }
 
-void launch_sketch_sample(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t sample_idx,
-                          uint32_t* out, hipStream_t s) {
+void launch_sketch_sample(int gbytes, const unsigned char* slab, const Geo& g, uint32_t sample_idx, uint32_t* out,
+                          hipStream_t s) {
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_sketch_sample<uint32_t>, dim3(1), dim3(kWave), 0, s, alpha, (const uint32_t*)gamma, g,
-                       sample_idx, out);
+    hipLaunchKernelGGL(k_sketch_sample<uint32_t>, dim3(1), dim3(kWave), 0, s, slab, g, sample_idx, out);
   else
-    hipLaunchKernelGGL(k_sketch_sample<uint64_t>, dim3(1), dim3(kWave), 0, s, alpha, (const uint64_t*)gamma, g,
-                       sample_idx, out);
+    hipLaunchKernelGGL(k_sketch_sample<uint64_t>, dim3(1), dim3(kWave), 0, s, slab, g, sample_idx, out);
 }
-void launch_xor_range(int gbytes, uint64_t* dst_a, void* dst_g, const uint64_t* src_a, const void* src_g,
-                      const Geo& g, uint32_t nsk, uint32_t b0, uint32_t cnt, bool with_det, hipStream_t s) {
+void launch_xor_range(int gbytes, unsigned char* dst, const unsigned char* src, const Geo& g, uint32_t nsk, uint32_t b0,
+                      uint32_t cnt, bool with_det, hipStream_t s) {
   const size_t total = (size_t)nsk * (cnt + (with_det ? 1 : 0));
   if (!total) return;
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_xor_range<uint32_t>, dim3(grid_for(total, 256)), dim3(256), 0, s, dst_a, (uint32_t*)dst_g,
-                       src_a, (const uint32_t*)src_g, g, nsk, b0, cnt, with_det ? 1 : 0);
+    hipLaunchKernelGGL(k_xor_range<uint32_t>, dim3(grid_for(total, 256)), dim3(256), 0, s, dst, src, g, nsk, b0, cnt,
+                       with_det ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_xor_range<uint64_t>, dim3(grid_for(total, 256)), dim3(256), 0, s, dst_a, (uint64_t*)dst_g,
-                       src_a, (const uint64_t*)src_g, g, nsk, b0, cnt, with_det ? 1 : 0);
+    hipLaunchKernelGGL(k_xor_range<uint64_t>, dim3(grid_for(total, 256)), dim3(256), 0, s, dst, src, g, nsk, b0, cnt,
+                       with_det ? 1 : 0);
 }
 
 template <int MODE>
-static void launch_packed(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
-                          const void* packed, hipStream_t s) {
+static void launch_packed(int gbytes, unsigned char* slab, const Geo& g, uint32_t v0, uint32_t nv, const void* packed,
+                          hipStream_t s) {
   const size_t total = (size_t)nv * g.nb;
   if (!total) return;
   unsigned char* p = (unsigned char*)packed;
   if (gbytes == 4)
-    hipLaunchKernelGGL((k_packed<uint32_t, MODE>), dim3(grid_for(total, 256)), dim3(256), 0, s, alpha,
-                       (uint32_t*)gamma, g, v0, nv, p);
+    hipLaunchKernelGGL((k_packed<uint32_t, MODE>), dim3(grid_for(total, 256)), dim3(256), 0, s, slab, g, v0, nv, p);
   else
-    hipLaunchKernelGGL((k_packed<uint64_t, MODE>), dim3(grid_for(total, 256)), dim3(256), 0, s, alpha,
-                       (uint64_t*)gamma, g, v0, nv, p);
+    hipLaunchKernelGGL((k_packed<uint64_t, MODE>), dim3(grid_for(total, 256)), dim3(256), 0, s, slab, g, v0, nv, p);
 }
-void launch_merge_packed(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
-                         const void* packed, hipStream_t s) {
-  launch_packed<0>(gbytes, alpha, gamma, g, v0, nv, packed, s);
+void launch_merge_packed(int gbytes, unsigned char* slab, const Geo& g, uint32_t v0, uint32_t nv, const void* packed,
+                         hipStream_t s) {
+  launch_packed<0>(gbytes, slab, g, v0, nv, packed, s);
 }
-void launch_unpack(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
-                   const void* packed, hipStream_t s) {
-  launch_packed<1>(gbytes, alpha, gamma, g, v0, nv, packed, s);
+void launch_unpack(int gbytes, unsigned char* slab, const Geo& g, uint32_t v0, uint32_t nv, const void* packed,
+                   hipStream_t s) {
+  launch_packed<1>(gbytes, slab, g, v0, nv, packed, s);
 }
-void launch_pack(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
-                 void* packed, hipStream_t s) {
-  launch_packed<2>(gbytes, const_cast<uint64_t*>(alpha), const_cast<void*>(gamma), g, v0, nv, packed, s);
+void launch_pack(int gbytes, const unsigned char* slab, const Geo& g, uint32_t v0, uint32_t nv, void* packed,
+                 hipStream_t s) {
+  launch_packed<2>(gbytes, const_cast<unsigned char*>(slab), g, v0, nv, packed, s);
 }
 
 }  // namespace gz

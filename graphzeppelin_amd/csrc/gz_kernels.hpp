@@ -40,7 +40,7 @@ void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);
 // (64-bit indices) is non-null.
 void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
                          uint32_t grid_cap, uint32_t* work_counter, const uint32_t* dsts, const uint64_t* raw_idx,
-                         uint64_t* alpha, void* gamma, const Geo& g, const uint64_t* col_bf, hipStream_t s);
+                         unsigned char* slab, const Geo& g, const uint64_t* col_bf, hipStream_t s);
 size_t update_tiles_lds_bytes(const Geo& g, int gbytes);
 
 // ---- grouping of a device-resident stream of (src,dst) pairs into per-vertex tiles ----
@@ -85,14 +85,14 @@ struct RoundBufs {
   uint64_t* samp;        // n      sample record per supernode root
 };
 // Writes rb.samp: a sample record per vertex.
-void launch_sample_round0(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, const DsuDev& d,
-                          const RoundBufs& rb, hipStream_t s);
+void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
+                          hipStream_t s);
 // Number of reduce levels for n vertices (entries_per_level gets a bound on each level's entries).
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
 // Queue round `round` (>= 1); every kernel of it is a no-op when d.mod[round - 1] == 0.
 // giant_ok: the vertex sketches are known to XOR to zero, so a supernode holding more than half of
 // the vertices may be summed from its complement.
-void launch_round(int gbytes, uint32_t round, const uint64_t* alpha, const void* gamma, const Geo& g,
+void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s);
 // Reset the round state of a query (before round 1).
 void launch_round_start(const RoundBufs& rb, hipStream_t s);
@@ -101,28 +101,25 @@ void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* q
                           hipStream_t s);
 
 // order-sensitive 64-bit digest of all n sketches (see k_digest)
-void launch_digest(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out, hipStream_t s);
+void launch_digest(int gbytes, const unsigned char* slab, const Geo& g, uint64_t* out, hipStream_t s);
 // Bucket-wise XOR of all n vertex sketches (out_a: nb u64, out_g: nb gamma words).
-void launch_sum_all(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint64_t* out_a, void* out_g,
-                    hipStream_t s);
-
-// adds sum_v 4k + 2*min(S, k*per_update) over per-vertex counts k = cnt[v] to *out
+void launch_sum_all(int gbytes, const unsigned char* slab, const Geo& g, uint64_t* out_a, void* out_g, hipStream_t s);
 
 // ---- standalone sketch ops ----
 // out[0] = code, out[1..2] = idx (lo, hi)
-void launch_sketch_sample(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t sample_idx,
-                          uint32_t* out, hipStream_t s);
+void launch_sketch_sample(int gbytes, const unsigned char* slab, const Geo& g, uint32_t sample_idx, uint32_t* out,
+                          hipStream_t s);
 // dst ^= src on buckets [b0, b0 + cnt) of `nsk` consecutive sketches, plus the deterministic bucket
-void launch_xor_range(int gbytes, uint64_t* dst_a, void* dst_g, const uint64_t* src_a, const void* src_g,
-                      const Geo& g, uint32_t nsk, uint32_t b0, uint32_t cnt, bool with_det, hipStream_t s);
-// XOR a packed (reference-format) bucket image of `nv` sketches starting at v0 into the planes
-void launch_merge_packed(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
-                         const void* packed, hipStream_t s);
-// pack `nv` sketches starting at v0 into the reference's byte image (AoS {u64, gamma})
-void launch_pack(int gbytes, const uint64_t* alpha, const void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
-                 void* packed, hipStream_t s);
-// unpack (overwrite) a packed image into the planes
-void launch_unpack(int gbytes, uint64_t* alpha, void* gamma, const Geo& g, uint32_t v0, uint32_t nv,
-                   const void* packed, hipStream_t s);
+void launch_xor_range(int gbytes, unsigned char* dst, const unsigned char* src, const Geo& g, uint32_t nsk, uint32_t b0,
+                      uint32_t cnt, bool with_det, hipStream_t s);
+// XOR a packed (reference-format) bucket image of `nv` sketches starting at v0 into the slab
+void launch_merge_packed(int gbytes, unsigned char* slab, const Geo& g, uint32_t v0, uint32_t nv, const void* packed,
+                         hipStream_t s);
+// pack `nv` sketches starting at v0 into the reference's byte image (records {u64 alpha, gamma})
+void launch_pack(int gbytes, const unsigned char* slab, const Geo& g, uint32_t v0, uint32_t nv, void* packed,
+                 hipStream_t s);
+// unpack (overwrite) a packed image into the slab
+void launch_unpack(int gbytes, unsigned char* slab, const Geo& g, uint32_t v0, uint32_t nv, const void* packed,
+                   hipStream_t s);
 
 }  // namespace gz

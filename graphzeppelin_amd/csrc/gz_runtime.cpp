@@ -29,7 +29,7 @@ void require_device() {
     throw GzError(GZ_ERR_DEVICE, "no HIP device is available: the HIP library needs a GPU");
 }
 
-static Geo make_geo(uint32_t nsk, uint64_t vector_len, uint64_t seed, size_t samples, size_t cols) {
+static Geo make_geo(uint32_t nsk, uint64_t vector_len, uint64_t seed, size_t samples, size_t cols, int gb) {
   if (vector_len < 1) throw GzError(GZ_ERR_INVALID_ARG, "vector length must be >= 1");
   if (samples < 1 || cols < 1) throw GzError(GZ_ERR_INVALID_ARG, "num_samples and cols_per_sample must be >= 1");
   Geo g{};
@@ -41,7 +41,7 @@ static Geo make_geo(uint32_t nsk, uint64_t vector_len, uint64_t seed, size_t sam
   g.cps = (uint32_t)cols;
   g.C = (uint32_t)(samples * cols);
   g.nb = g.C * g.D + 1;
-  g.stride = (g.nb + 31u) & ~31u;
+  g.row = (uint32_t)(((size_t)g.nb * (8 + gb) + 255) & ~(size_t)255);
   g.Dp = lds_pitch(g.D);
   g.seed = seed;
   g.chk_bf = xxh3_bitflip(seed);
@@ -64,11 +64,10 @@ static void check_gamma(int gb) {
 GzSketch::GzSketch(uint64_t vector_len, uint64_t seed, size_t samples, size_t cols, int gamma_bytes) : gb(gamma_bytes) {
   check_gamma(gb);
   require_device();
-  geo = make_geo(1, vector_len, seed, samples, cols);
+  geo = make_geo(1, vector_len, seed, samples, cols, gb);
   if (update_tiles_lds_bytes(geo, gb) > 64 * 1024) throw GzError(GZ_ERR_INVALID_ARG, "sketch too large for one workgroup's LDS");
   GZ_HIP(hipStreamCreate(&st));
-  alpha.alloc(geo.stride);
-  gamma.alloc((size_t)geo.stride * gb);
+  slab.alloc(geo.row);
   out.alloc(4);
   auto bf = column_bitflips(geo);
   col_bf.alloc(bf.size());
@@ -82,8 +81,7 @@ GzSketch::~GzSketch() {
   }
 }
 void GzSketch::zero() {  // src/sketch.cpp:86-92
-  launch_fill_u64(alpha.p, 0, geo.stride, st);
-  launch_fill_u64(reinterpret_cast<uint64_t*>(gamma.p), 0, (size_t)geo.stride * gb / 8, st);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(slab.p), 0, geo.row / 4, st);
   GZ_HIP(hipStreamSynchronize(st));
   sample_idx = 0;
 }
@@ -98,14 +96,14 @@ void GzSketch::update_batch(const uint64_t* idx, size_t count) {  // src/sketch.
   tiles.ensure(nt);
   GZ_HIP(hipMemcpyAsync(raw.p, idx, count * 8, hipMemcpyHostToDevice, st));
   GZ_HIP(hipMemcpyAsync(tiles.p, ht.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
-  launch_update_tiles(gb, tiles.p, nullptr, (uint32_t)nt, 0, nullptr, nullptr, raw.p, alpha.p, gamma.p, geo, col_bf.p, st);
+  launch_update_tiles(gb, tiles.p, nullptr, (uint32_t)nt, 0, nullptr, nullptr, raw.p, slab.p, geo, col_bf.p, st);
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipStreamSynchronize(st));
 }
 int GzSketch::sample(uint64_t* idx) {  // src/sketch.cpp:94-116
   if (sample_idx >= geo.samples) throw GzError(GZ_ERR_OUT_OF_SAMPLES, "sketch cannot be sampled more times");
   const uint32_t s = (uint32_t)sample_idx++;
-  launch_sketch_sample(gb, alpha.p, gamma.p, geo, s, out.p, st);
+  launch_sketch_sample(gb, slab.p, geo, s, out.p, st);
   uint32_t h[3];
   GZ_HIP(hipMemcpyAsync(h, out.p, 12, hipMemcpyDeviceToHost, st));
   GZ_HIP(hipStreamSynchronize(st));
@@ -115,7 +113,7 @@ int GzSketch::sample(uint64_t* idx) {  // src/sketch.cpp:94-116
 void GzSketch::merge(const GzSketch& o) {  // src/sketch.cpp:149-154
   if (o.geo.nb != geo.nb || o.gb != gb) throw GzError(GZ_ERR_INVALID_ARG, "sketch shapes differ");
   GZ_HIP(hipStreamSynchronize(o.st));
-  launch_xor_range(gb, alpha.p, gamma.p, o.alpha.p, o.gamma.p, geo, 1, 0, geo.nb - 1, true, st);
+  launch_xor_range(gb, slab.p, o.slab.p, geo, 1, 0, geo.nb - 1, true, st);
   GZ_HIP(hipStreamSynchronize(st));
 }
 void GzSketch::range_merge(const GzSketch& o, size_t start, size_t ns) {  // src/sketch.cpp:156-179
@@ -127,20 +125,20 @@ void GzSketch::range_merge(const GzSketch& o, size_t start, size_t ns) {  // src
   sample_idx = std::max(sample_idx, start);
   GZ_HIP(hipStreamSynchronize(o.st));
   const uint32_t b0 = (uint32_t)(start * geo.cps * geo.D), cnt = (uint32_t)(ns * geo.cps * geo.D);
-  launch_xor_range(gb, alpha.p, gamma.p, o.alpha.p, o.gamma.p, geo, 1, b0, cnt, true, st);
+  launch_xor_range(gb, slab.p, o.slab.p, geo, 1, b0, cnt, true, st);
   GZ_HIP(hipStreamSynchronize(st));
 }
 void GzSketch::merge_raw(const void* rawb) {  // src/sketch.cpp:181-186
   packed.ensure(bytes());
   GZ_HIP(hipMemcpyAsync(packed.p, rawb, bytes(), hipMemcpyHostToDevice, st));
-  launch_merge_packed(gb, alpha.p, gamma.p, geo, 0, 1, packed.p, st);
+  launch_merge_packed(gb, slab.p, geo, 0, 1, packed.p, st);
   GZ_HIP(hipStreamSynchronize(st));
 }
 void GzSketch::serialize(void* o, size_t nbytes) const {  // src/sketch.cpp:188-190
   if (nbytes < bytes()) throw GzError(GZ_ERR_INVALID_ARG, "output buffer smaller than bucket_array_bytes()");
   auto* self = const_cast<GzSketch*>(this);
   self->packed.ensure(bytes());
-  launch_pack(gb, alpha.p, gamma.p, geo, 0, 1, self->packed.p, st);
+  launch_pack(gb, slab.p, geo, 0, 1, self->packed.p, st);
   GZ_HIP(hipMemcpyAsync(o, packed.p, bytes(), hipMemcpyDeviceToHost, st));
   GZ_HIP(hipStreamSynchronize(st));
 }
@@ -148,7 +146,7 @@ void GzSketch::deserialize(const void* in, size_t nbytes) {  // src/sketch.cpp:2
   if (nbytes < bytes()) throw GzError(GZ_ERR_INVALID_ARG, "input buffer smaller than bucket_array_bytes()");
   packed.ensure(bytes());
   GZ_HIP(hipMemcpyAsync(packed.p, in, bytes(), hipMemcpyHostToDevice, st));
-  launch_unpack(gb, alpha.p, gamma.p, geo, 0, 1, packed.p, st);
+  launch_unpack(gb, slab.p, geo, 0, 1, packed.p, st);
   GZ_HIP(hipStreamSynchronize(st));
 }
 
@@ -163,9 +161,9 @@ static size_t default_cap_pairs(uint32_t n, size_t desired) {
 uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint64_t len = calc_vector_length(n);
   const size_t D = calc_bkt_per_col(len), C = calc_cc_samples(n, c.sketches_factor);
-  const uint64_t nb = C * D + 1, stride = (nb + 31) & ~31ull;
+  const uint64_t nb = C * D + 1;
   const int gbytes = c.gamma_bytes ? c.gamma_bytes : 4;
-  const uint64_t slab = (uint64_t)n * stride * (8 + gbytes);
+  const uint64_t slab = (uint64_t)n * ((nb * (8 + gbytes) + 255) & ~255ull);
   const size_t desired = (size_t)(nb * (8 + gbytes) / 4 * c.batch_factor);
   const uint64_t cap = c.flush_threshold ? c.flush_threshold : default_cap_pairs(n, desired);
   const uint32_t t = c.tile ? c.tile : 1024;
@@ -186,7 +184,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   gb = cfg.gamma_bytes;
   check_gamma(gb);
   require_device();
-  geo = make_geo(n, calc_vector_length(n), seed, calc_cc_samples(n, cfg.sketches_factor), 1);
+  geo = make_geo(n, calc_vector_length(n), seed, calc_cc_samples(n, cfg.sketches_factor), 1, gb);
   if (update_tiles_lds_bytes(geo, gb) > 64 * 1024) throw GzError(GZ_ERR_INVALID_ARG, "sketch too large for LDS");
   tile = cfg.tile ? cfg.tile : 1024;
   const size_t lds = update_tiles_lds_bytes(geo, gb);
@@ -195,10 +193,8 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   GZ_HIP(hipStreamCreate(&st));
 
   // sketch slab
-  alpha.alloc((size_t)n * geo.stride);
-  gamma.alloc((size_t)n * geo.stride * gb);
-  launch_fill_u64(alpha.p, 0, alpha.n, st);
-  launch_fill_u64(reinterpret_cast<uint64_t*>(gamma.p), 0, gamma.n / 8, st);
+  slab.alloc((size_t)n * geo.row);
+  launch_fill_u32(reinterpret_cast<uint32_t*>(slab.p), 0, slab.n / 4, st);
   auto bf = column_bitflips(geo);
   col_bf.alloc(bf.size());
   GZ_HIP(hipMemcpyAsync(col_bf.p, bf.data(), bf.size() * 8, hipMemcpyHostToDevice, st));
@@ -375,7 +371,7 @@ void GzAlg::submit_stage_locked() {
   GZ_HIP(hipMemcpyAsync(s.dev.p + doff, s.host.p + doff, s.used * 4, hipMemcpyHostToDevice, st));
   if (prof) prof_begin(0, batch_alg_bytes(s.ent_cnt), false, s.used);
   launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
-                      reinterpret_cast<uint32_t*>(s.dev.p + doff), nullptr, alpha.p, gamma.p, geo, col_bf.p, st);
+                      reinterpret_cast<uint32_t*>(s.dev.p + doff), nullptr, slab.p, geo, col_bf.p, st);
   if (prof) prof_end();
   GZ_HIP(hipGetLastError());
   if (!s.ev) GZ_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
@@ -417,7 +413,7 @@ void GzAlg::plain_launch(uint32_t src, const uint32_t* dsts, size_t k) {
   GZ_HIP(hipMemcpyAsync(s.dev.p, s.host.p, need, hipMemcpyHostToDevice, st));
   if (prof) prof_begin(0, batch_alg_bytes(std::vector<uint32_t>{(uint32_t)k}), false, k);
   launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
-                      reinterpret_cast<uint32_t*>(s.dev.p + tbytes), nullptr, alpha.p, gamma.p, geo, col_bf.p, st);
+                      reinterpret_cast<uint32_t*>(s.dev.p + tbytes), nullptr, slab.p, geo, col_bf.p, st);
   if (prof) prof_end();
   GZ_HIP(hipGetLastError());
   if (!s.ev) GZ_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
@@ -433,7 +429,7 @@ void GzAlg::apply_raw(uint32_t src, const void* rawb) {
   DevBuf<uint8_t> tmp;
   tmp.alloc(sketch_bytes());
   GZ_HIP(hipMemcpyAsync(tmp.p, rawb, sketch_bytes(), hipMemcpyHostToDevice, st));
-  launch_merge_packed(gb, alpha.p, gamma.p, geo, src, 1, tmp.p, st);
+  launch_merge_packed(gb, slab.p, geo, src, 1, tmp.p, st);
   GZ_HIP(hipStreamSynchronize(st));
   zero_sum = 0;  // a raw delta on one vertex may break the zero sum
 }
@@ -486,7 +482,7 @@ void GzAlg::flush_pending() {
   const FlushBytes fb{prof_bytes.p, sketch_bytes(), (uint64_t)(geo.C + 1) * (8 + gb)};
   launch_group(pending.p, pending_n, n, tile, gbufs, st, prof ? &fb : nullptr);
   if (prof) prof_begin(0, 0, true, 2 * pending_n);
-  launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, work_counter.p, dst_arr.p, nullptr, alpha.p, gamma.p,
+  launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, work_counter.p, dst_arr.p, nullptr, slab.p,
                       geo, col_bf.p, st);
   if (prof) prof_end();
   GZ_HIP(hipGetLastError());
@@ -587,7 +583,7 @@ bool GzAlg::sketches_sum_to_zero() {
     DevBuf<uint64_t> sa, sg;
     sa.alloc(geo.nb);
     sg.alloc(geo.nb);
-    launch_sum_all(gb, alpha.p, gamma.p, geo, sa.p, sg.p, st);
+    launch_sum_all(gb, slab.p, geo, sa.p, sg.p, st);
     std::vector<uint64_t> ha(geo.nb), hg(geo.nb);
     GZ_HIP(hipMemcpyAsync(ha.data(), sa.p, geo.nb * 8, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipMemcpyAsync(hg.data(), sg.p, geo.nb * (size_t)gb, hipMemcpyDeviceToHost, st));
@@ -607,8 +603,8 @@ void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
   launch_dsu_reset(dsu, st);
   launch_fill_u32(r_cnt.p, 0, n, st);
   launch_round_start(rb, st);
-  launch_sample_round0(gb, alpha.p, gamma.p, geo, dsu, rb, st);
-  for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
+  launch_sample_round0(gb, slab.p, geo, dsu, rb, st);
+  for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
 }
 
 hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
@@ -662,7 +658,7 @@ void GzAlg::boruvka() {
       if (upto >= last_round)
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
       const uint32_t hi = std::min(next + 1, last_round);
-      for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, alpha.p, gamma.p, geo, dsu, rb, giant_ok, st);
+      for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
       GZ_HIP(hipGetLastError());
       next = hi + 1;
     }
@@ -782,7 +778,7 @@ void GzAlg::dump(uint32_t v0, uint32_t count, void* outp) {
   tmp.alloc((size_t)std::min(chunk, count ? count : 1) * sb);
   for (uint32_t i = 0; i < count; i += chunk) {
     const uint32_t c = std::min(chunk, count - i);
-    launch_pack(gb, alpha.p, gamma.p, geo, v0 + i, c, tmp.p, st);
+    launch_pack(gb, slab.p, geo, v0 + i, c, tmp.p, st);
     GZ_HIP(hipMemcpyAsync((uint8_t*)outp + (size_t)i * sb, tmp.p, (size_t)c * sb, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
   }
@@ -792,7 +788,7 @@ uint64_t GzAlg::digest() {
   flush_all();
   DevBuf<uint64_t> d;
   d.alloc(1);
-  launch_digest(gb, alpha.p, gamma.p, geo, d.p, st);
+  launch_digest(gb, slab.p, geo, d.p, st);
   uint64_t h = 0;
   GZ_HIP(hipMemcpyAsync(&h, d.p, 8, hipMemcpyDeviceToHost, st));
   GZ_HIP(hipStreamSynchronize(st));
@@ -807,7 +803,7 @@ void GzAlg::load_packed(uint32_t v0, uint32_t count, const void* in) {
   for (uint32_t i = 0; i < count; i += chunk) {
     const uint32_t c = std::min(chunk, count - i);
     GZ_HIP(hipMemcpyAsync(tmp.p, (const uint8_t*)in + (size_t)i * sb, (size_t)c * sb, hipMemcpyHostToDevice, st));
-    launch_unpack(gb, alpha.p, gamma.p, geo, v0 + i, c, tmp.p, st);
+    launch_unpack(gb, slab.p, geo, v0 + i, c, tmp.p, st);
     GZ_HIP(hipStreamSynchronize(st));
   }
   zero_sum = 0;

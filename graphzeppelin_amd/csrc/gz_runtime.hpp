@@ -98,8 +98,8 @@ class GzSketch {
   int gb;
   size_t sample_idx = 0;
   hipStream_t st = nullptr;
-  DevBuf<uint64_t> alpha, col_bf, raw;
-  DevBuf<uint8_t> gamma;
+  DevBuf<uint8_t> slab;  // packed bucket records (see Geo::row)
+  DevBuf<uint64_t> col_bf, raw;
   DevBuf<uint32_t> dsts, out;
   DevBuf<Tile> tiles;
   DevBuf<uint8_t> packed;
@@ -133,8 +133,8 @@ class GzAlg {
   hipStream_t st = nullptr;
 
   // sketch slab
-  DevBuf<uint64_t> alpha, col_bf;
-  DevBuf<uint8_t> gamma;
+  DevBuf<uint8_t> slab;  // packed bucket records, one 256-B aligned row per vertex
+  DevBuf<uint64_t> col_bf;
   // DSU / forest
   DevBuf<uint32_t> parent, flags, labels, qa, qb;
   DevBuf<uint64_t> sf_edge;
