@@ -254,59 +254,57 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
 // Sketch update: Sketch::update over a batch (src/sketch.cpp:69-83) fused with the delta merge
 // (src/cc_sketch_alg.cpp:106-118, src/sketch.cpp:149-154).
 //
-// One workgroup per tile (<= tile-size destinations of one vertex).  The tile's delta sketch
-// lives in LDS; each thread hashes (update, column) pairs, so consecutive lanes XOR into different
-// columns at the same time (column pitch Dp = 1 mod 16 spreads them over the LDS banks) instead of
-// all hitting depth 0 of one column.  The deterministic bucket is kept
-// in registers per lane and reduced once per tile.  The write-back is a coalesced
-// read-xor-write of the vertex's bucket planes, or device-scope atomic XOR when the vertex is
-// split across tiles of the same launch.  Workgroups walk the tiles grid-stride, so a persistent
-// grid can cover a tile count that only the device knows.
+// One wave (a 64-thread workgroup) per tile of <= tile-size destinations of one vertex, so a CU
+// keeps up to 32 tiles in flight: most tiles of a flush hold a handful of updates, and a tile's
+// time is the latency of its few dependent memory round trips, not bandwidth.  Lanes take
+// (update, column) pairs; the tile's delta for depths 0..kLdsDepth-1 of every column lives in LDS
+// (column pitch kLdsPitch = 9, odd, so lanes on consecutive columns hit different banks), and the
+// rare deeper contributions (one pair in 2^kLdsDepth) are XORed straight into the vertex's records
+// with device-scope atomics.  The deterministic bucket is kept in registers and reduced once per
+// tile.  The write-back is a read-xor-write of the touched records, or device-scope atomic XOR
+// when the vertex is split across tiles of the same launch.  Workgroups walk the tiles
+// grid-stride (no shared counter), so a persistent grid can cover a tile count that only the
+// device knows.
 // ================================================================================================
-template <class G, bool RAW, int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
+constexpr uint32_t kLdsDepth = 8, kLdsPitch = 9;
+
+template <class G, bool RAW>
+__global__ void __launch_bounds__(kWave)
 k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntiles_dev, uint32_t ntiles_host,
-               uint32_t* __restrict__ work_counter, const uint32_t* __restrict__ dsts,
-               const uint64_t* __restrict__ raw_idx, unsigned char* __restrict__ slab, Geo g,
-               const uint64_t* __restrict__ col_bf) {
+               const uint32_t* __restrict__ dsts, const uint64_t* __restrict__ raw_idx,
+               unsigned char* __restrict__ slab, Geo g, const uint64_t* __restrict__ col_bf) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t C = g.C, D = g.D, Dp = g.Dp;
-  const uint32_t lds_n = C * Dp + 1;  // + deterministic bucket at C*Dp
-  const uint32_t nb = C * D + 1;
+  const uint32_t C = g.C, D = g.D;
+  const uint32_t det_slot = C * kLdsPitch;
+  const uint32_t lds_n = det_slot + 1;
+  constexpr uint32_t RB = 8 + sizeof(G);
   uint64_t* s_bf = reinterpret_cast<uint64_t*>(smem);
   uint64_t* s_a = s_bf + C;
   G* s_g = reinterpret_cast<G*>(s_a + lds_n);
-  __shared__ uint32_t s_tile;
+  const uint32_t lane = threadIdx.x;
 
-  for (uint32_t i = threadIdx.x; i < C; i += BLOCK) s_bf[i] = col_bf[i];
-  // The delta is zeroed once here; afterwards the write-back clears every entry it reads, so the
-  // next tile starts from zero without a separate pass.  LDS entries with depth >= D (pitch
-  // padding) are never written.
-  for (uint32_t i = threadIdx.x; i < lds_n; i += BLOCK) { s_a[i] = 0; s_g[i] = 0; }
+  for (uint32_t i = lane; i < C; i += kWave) s_bf[i] = col_bf[i];
+  // The delta is zeroed once here; afterwards the write-back clears every entry it reads.
+  for (uint32_t i = lane; i < lds_n; i += kWave) {
+    s_a[i] = 0;
+    s_g[i] = 0;
+  }
   const uint32_t ntiles = ntiles_dev ? *ntiles_dev : ntiles_host;
-  const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint64_t depth_cap = 1ull << D;
-  // (update, column) pairs are spread over the threads: thread t starts at pair t = i0 * C + c0
-  // and advances BLOCK pairs at a time (di updates and dc columns, with a carry).
-  const uint32_t i0 = threadIdx.x / C, c0 = threadIdx.x % C;
-  const uint32_t di = BLOCK / C, dc = BLOCK % C;
-  // Tiles are taken grid-stride (t, t + gridDim.x, ...): no shared counter, so no device-wide
-  // atomic on one hot address per tile.
-  if (threadIdx.x == 0) s_tile = blockIdx.x;
+  // pair p = i * C + c: lane starts at pair `lane` and advances 64 pairs (di updates, dc columns)
+  const uint32_t i0 = lane / C, c0 = lane % C;
+  const uint32_t di = kWave / C, dc = kWave % C;
   __syncthreads();
 
-  while (true) {
-    const uint32_t t = s_tile;
-    if (t >= ntiles) break;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tiles[t];
+    unsigned char* row = slab + (size_t)tl.vertex * g.row;
 
     uint64_t det_a = 0;
     G det_g = 0;
-    // One thread per (update, column) pair, so a tile of a few updates still keeps every lane
-    // busy; the column-0 thread of each update also folds it into the deterministic bucket.
     const uint32_t npairs = tl.count * C;
     uint32_t i = i0, c = c0;
-    for (uint32_t pr = threadIdx.x; pr < npairs; pr += BLOCK) {
+    for (uint32_t pr = lane; pr < npairs; pr += kWave) {
       const uint64_t idx = RAW ? raw_idx[tl.begin + i] : concat_pairing(tl.vertex, dsts[tl.begin + i]);
       const uint64_t pre = xxh3_prep(idx);
       const G chk = (G)xxh3_mix(pre, g.chk_bf);
@@ -315,11 +313,15 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
         det_g ^= chk;
       }
       const uint64_t h = xxh3_mix(pre, s_bf[c]);
-      const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);
+      const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);  // == D: no bucket
       if (depth < D) {
-        const uint32_t li = c * Dp + depth;
-        lds_xor(s_a + li, idx);
-        lds_xor(s_g + li, chk);
+        if (depth < kLdsDepth) {
+          const uint32_t li = c * kLdsPitch + depth;
+          lds_xor(s_a + li, idx);
+          lds_xor(s_g + li, chk);
+        } else {
+          xor_rec_atomic<G>(row + (size_t)(c * D + depth) * RB, idx, chk);
+        }
       }
       i += di;
       c += dc;
@@ -331,39 +333,31 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     det_a = wave_xor_u64(det_a);
     det_g = (G)wave_xor_u64((uint64_t)det_g);
     if (lane == 0 && (det_a | (uint64_t)det_g)) {
-      lds_xor(s_a + C * Dp, det_a);
-      lds_xor(s_g + C * Dp, det_g);
+      s_a[det_slot] ^= det_a;
+      s_g[det_slot] ^= det_g;
     }
     __syncthreads();
-    // next tile, read after the barrier below
-    if (threadIdx.x == 0) s_tile = t + gridDim.x;
 
-    // write-back: read-xor-write of the vertex's records, four buckets per thread in flight
-    unsigned char* row = slab + (size_t)tl.vertex * g.row;
-    constexpr uint32_t RB = 8 + sizeof(G);
-    for (uint32_t b0 = 0; b0 < nb; b0 += 4 * BLOCK) {
+    // write-back: read-xor-write of the touched records, four LDS entries per lane in flight
+    for (uint32_t e0 = 0; e0 < lds_n; e0 += 4 * kWave) {
       uint64_t da[4], ga[4];
       G dg[4], gg[4];
       uint32_t bi[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const uint32_t b = b0 + k * BLOCK + threadIdx.x;
-        bi[k] = b;
+        const uint32_t e = e0 + k * kWave + lane;
         da[k] = 0;
         dg[k] = 0;
-        if (b < nb) {
-          uint32_t li;
-          if (b == nb - 1) {
-            li = C * Dp;
-          } else {
-            const uint32_t c = b / D;
-            li = c * Dp + (b - c * D);
-          }
-          da[k] = s_a[li];
-          dg[k] = s_g[li];
+        bi[k] = 0;
+        if (e < lds_n) {
+          const uint32_t cc = e / kLdsPitch, d = e - cc * kLdsPitch;
+          // entries at the pitch padding (d == kLdsDepth) or at depth >= D are never written
+          bi[k] = e == det_slot ? C * D : cc * D + d;
+          da[k] = s_a[e];
+          dg[k] = s_g[e];
           if ((da[k] | (uint64_t)dg[k]) != 0) {
-            s_a[li] = 0;
-            s_g[li] = 0;
+            s_a[e] = 0;
+            s_g[e] = 0;
           }
         }
       }
@@ -1173,15 +1167,13 @@ void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) {
 }
 
 size_t update_tiles_lds_bytes(const Geo& g, int gbytes) {
-  const size_t lds_n = (size_t)g.C * g.Dp + 1;
+  const size_t lds_n = (size_t)g.C * kLdsPitch + 1;
   return g.C * 8 + lds_n * 8 + lds_n * (size_t)gbytes;
 }
 
 void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
-                         uint32_t grid_cap, uint32_t* work_counter, const uint32_t* dsts, const uint64_t* raw_idx,
-                         unsigned char* slab, const Geo& g, const uint64_t* col_bf, hipStream_t s) {
-  constexpr int B = 256;
-  (void)work_counter;
+                         uint32_t grid_cap, const uint32_t* dsts, const uint64_t* raw_idx, unsigned char* slab,
+                         const Geo& g, const uint64_t* col_bf, hipStream_t s) {
   // tile count known on the host: one workgroup per tile (capped, then grid-stride);
   // known only on the device: a persistent grid of grid_cap workgroups
   uint32_t grid = ntiles_dev ? grid_cap : (grid_cap && ntiles_host > grid_cap ? grid_cap : ntiles_host);
@@ -1189,18 +1181,18 @@ void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_d
   const size_t lds = update_tiles_lds_bytes(g, gbytes);
   if (gbytes == 4) {
     if (raw_idx)
-      hipLaunchKernelGGL((k_update_tiles<uint32_t, true, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
+      hipLaunchKernelGGL((k_update_tiles<uint32_t, true>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
+                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
     else
-      hipLaunchKernelGGL((k_update_tiles<uint32_t, false, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
+      hipLaunchKernelGGL((k_update_tiles<uint32_t, false>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
+                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
   } else {
     if (raw_idx)
-      hipLaunchKernelGGL((k_update_tiles<uint64_t, true, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
+      hipLaunchKernelGGL((k_update_tiles<uint64_t, true>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
+                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
     else
-      hipLaunchKernelGGL((k_update_tiles<uint64_t, false, B>), dim3(grid), dim3(B), lds, s, tiles, ntiles_dev,
-                         ntiles_host, work_counter, dsts, raw_idx, slab, g, col_bf);
+      hipLaunchKernelGGL((k_update_tiles<uint64_t, false>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
+                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
   }
 }
 

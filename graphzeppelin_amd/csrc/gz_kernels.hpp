@@ -33,14 +33,14 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s);
 
 // ---- sketch update ----
-// Tile kernel.  `ntiles_dev` (device scalar) bounds the tile index when non-null (persistent grid
-// of grid_cap workgroups), otherwise `ntiles_host` (one workgroup per tile, at most grid_cap when
-// grid_cap > 0).  Workgroups walk tiles grid-stride.  `work_counter` is unused.
+// Tile kernel (one wave per tile).  `ntiles_dev` (device scalar) bounds the tile index when
+// non-null (persistent grid of grid_cap workgroups), otherwise `ntiles_host` (one workgroup per
+// tile, at most grid_cap when grid_cap > 0).  Workgroups walk tiles grid-stride.
 // Exactly one of `dsts` (destination vertex ids; index = concat(vertex, dst)) or `raw_idx`
 // (64-bit indices) is non-null.
 void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
-                         uint32_t grid_cap, uint32_t* work_counter, const uint32_t* dsts, const uint64_t* raw_idx,
-                         unsigned char* slab, const Geo& g, const uint64_t* col_bf, hipStream_t s);
+                         uint32_t grid_cap, const uint32_t* dsts, const uint64_t* raw_idx, unsigned char* slab,
+                         const Geo& g, const uint64_t* col_bf, hipStream_t s);
 size_t update_tiles_lds_bytes(const Geo& g, int gbytes);
 
 // ---- grouping of a device-resident stream of (src,dst) pairs into per-vertex tiles ----

@@ -96,7 +96,7 @@ void GzSketch::update_batch(const uint64_t* idx, size_t count) {  // src/sketch.
   tiles.ensure(nt);
   GZ_HIP(hipMemcpyAsync(raw.p, idx, count * 8, hipMemcpyHostToDevice, st));
   GZ_HIP(hipMemcpyAsync(tiles.p, ht.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, st));
-  launch_update_tiles(gb, tiles.p, nullptr, (uint32_t)nt, 0, nullptr, nullptr, raw.p, slab.p, geo, col_bf.p, st);
+  launch_update_tiles(gb, tiles.p, nullptr, (uint32_t)nt, 0, nullptr, raw.p, slab.p, geo, col_bf.p, st);
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipStreamSynchronize(st));
 }
@@ -188,7 +188,8 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   if (update_tiles_lds_bytes(geo, gb) > 64 * 1024) throw GzError(GZ_ERR_INVALID_ARG, "sketch too large for LDS");
   tile = cfg.tile ? cfg.tile : 1024;
   const size_t lds = update_tiles_lds_bytes(geo, gb);
-  const size_t per_cu = std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / lds));
+  // one-wave workgroups: up to 32 per CU (8 waves on each of 4 SIMDs), fewer if LDS runs out
+  const size_t per_cu = std::max<size_t>(1, std::min<size_t>(32, (160 * 1024) / lds));
   grid_cap = (uint32_t)(device_cu_count() * per_cu);
   GZ_HIP(hipStreamCreate(&st));
 
@@ -241,7 +242,6 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   g_cnt.alloc(n); g_off.alloc(n + 1); g_toff.alloc(n + 1); g_cursor.alloc(n); g_totals.alloc(4);
   g_partials.alloc(group_partials_len(n));
   g_tiles.alloc(2 * cap_pairs / tile + n + 1);
-  work_counter.alloc(1);
   gbufs = GroupBufs{g_cnt.p, g_off.p, g_toff.p, g_cursor.p, g_partials.p, g_totals.p, dst_arr.p, g_tiles.p,
                     flags.p + kFlagBadInput};
 
@@ -370,7 +370,7 @@ void GzAlg::submit_stage_locked() {
   GZ_HIP(hipMemcpyAsync(s.dev.p, s.host.p, nt * sizeof(Tile), hipMemcpyHostToDevice, st));
   GZ_HIP(hipMemcpyAsync(s.dev.p + doff, s.host.p + doff, s.used * 4, hipMemcpyHostToDevice, st));
   if (prof) prof_begin(0, batch_alg_bytes(s.ent_cnt), false, s.used);
-  launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
+  launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0,
                       reinterpret_cast<uint32_t*>(s.dev.p + doff), nullptr, slab.p, geo, col_bf.p, st);
   if (prof) prof_end();
   GZ_HIP(hipGetLastError());
@@ -412,7 +412,7 @@ void GzAlg::plain_launch(uint32_t src, const uint32_t* dsts, size_t k) {
   std::memcpy(s.host.p + tbytes, dsts, k * 4);
   GZ_HIP(hipMemcpyAsync(s.dev.p, s.host.p, need, hipMemcpyHostToDevice, st));
   if (prof) prof_begin(0, batch_alg_bytes(std::vector<uint32_t>{(uint32_t)k}), false, k);
-  launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0, nullptr,
+  launch_update_tiles(gb, reinterpret_cast<Tile*>(s.dev.p), nullptr, (uint32_t)nt, 0,
                       reinterpret_cast<uint32_t*>(s.dev.p + tbytes), nullptr, slab.p, geo, col_bf.p, st);
   if (prof) prof_end();
   GZ_HIP(hipGetLastError());
@@ -482,7 +482,7 @@ void GzAlg::flush_pending() {
   const FlushBytes fb{prof_bytes.p, sketch_bytes(), (uint64_t)(geo.C + 1) * (8 + gb)};
   launch_group(pending.p, pending_n, n, tile, gbufs, st, prof ? &fb : nullptr);
   if (prof) prof_begin(0, 0, true, 2 * pending_n);
-  launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, work_counter.p, dst_arr.p, nullptr, slab.p,
+  launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, dst_arr.p, nullptr, slab.p,
                       geo, col_bf.p, st);
   if (prof) prof_end();
   GZ_HIP(hipGetLastError());

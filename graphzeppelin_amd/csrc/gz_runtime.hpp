@@ -149,7 +149,7 @@ class GzAlg {
   // device stream ingest (pending pairs -> grouping -> tiles)
   size_t cap_pairs;
   size_t pending_n = 0;
-  DevBuf<uint32_t> pending, dst_arr, g_cnt, g_off, g_toff, g_cursor, g_totals, work_counter;
+  DevBuf<uint32_t> pending, dst_arr, g_cnt, g_off, g_toff, g_cursor, g_totals;
   DevBuf<uint64_t> g_partials;
   DevBuf<Tile> g_tiles;
   DevBuf<uint8_t> merged;
