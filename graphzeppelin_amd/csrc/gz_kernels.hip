@@ -136,6 +136,21 @@ __device__ inline uint32_t dsu_find(uint32_t* parent, uint32_t u) {
     p = ld_acq(parent + u);
   }
 }
+// find for kernels in which no thread links roots, with pointer jumping: each thread publishes how
+// far it got in its own entry, so a thread that reaches an ancestor mid-walk skips whatever that
+// ancestor's thread has already walked.  All threads walking at once shorten every chain
+// geometrically instead of hop by hop.  Every stored pointer is an ancestor, so any mix of
+// old and new values is still a path to the root.
+__device__ inline uint32_t dsu_find_jump(uint32_t* parent, uint32_t u) {
+  uint32_t p = ld_acq(parent + u);
+  if (p == u) return u;
+  while (true) {
+    const uint32_t gp = ld_acq(parent + p);
+    if (gp == p) return p;
+    st_rlx(parent + u, gp);
+    p = gp;
+  }
+}
 // find for kernels in which no thread links roots (only compression writes, which keep every
 // pointer on the path to the root): cached loads are enough.
 __device__ inline uint32_t dsu_find_stable(uint32_t* parent, uint32_t u) {
@@ -728,12 +743,10 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
   const uint32_t v = in_walk ? (act ? act_list[i] : i) : 0;
   uint32_t r = 0;
   if (in_walk) {
-    r = dsu_find_stable(parent, v);
+    r = dsu_find_jump(parent, v);  // leaves parent[v] == r: the walked vertices end fully compressed
     root[v] = r;
   }
   __syncthreads();
-  // full path compression: every walked vertex now points at its root (no links in this kernel)
-  if (in_walk) parent[v] = r;
   const bool active = in_walk && r != gcur;  // members of the giant are not counted
   uint64_t todo = __ballot(active);
   while (todo) {
