@@ -136,21 +136,6 @@ __device__ inline uint32_t dsu_find(uint32_t* parent, uint32_t u) {
     p = ld_acq(parent + u);
   }
 }
-// find for kernels in which no thread links roots, with pointer jumping: each thread publishes how
-// far it got in its own entry, so a thread that reaches an ancestor mid-walk skips whatever that
-// ancestor's thread has already walked.  All threads walking at once shorten every chain
-// geometrically instead of hop by hop.  Every stored pointer is an ancestor, so any mix of
-// old and new values is still a path to the root.
-__device__ inline uint32_t dsu_find_jump(uint32_t* parent, uint32_t u) {
-  uint32_t p = ld_acq(parent + u);
-  if (p == u) return u;
-  while (true) {
-    const uint32_t gp = ld_acq(parent + p);
-    if (gp == p) return p;
-    st_rlx(parent + u, gp);
-    p = gp;
-  }
-}
 // find for kernels in which no thread links roots (only compression writes, which keep every
 // pointer on the path to the root): cached loads are enough.
 __device__ inline uint32_t dsu_find_stable(uint32_t* parent, uint32_t u) {
@@ -656,13 +641,17 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot,
 //   kStGiantFirst     the root elected giant first in this query (any member identifies the giant,
 //                     which only grows), or kNoRoot
 //   kStActMode        1 when this round walks the active list instead of every vertex
-// Once a giant exists, the active list of round r is the member list regrouped in round r - 1:
-// every vertex outside the giant.  Vertices that joined the giant are dropped as they are met.
+// Round 1 walks every vertex in id order.  Round r >= 2 walks the member list regrouped in round
+// r - 1: every vertex outside the giant (all of them before one is elected), grouped by the
+// previous round's roots, so the members of a new supernode mostly sit next to each other and a
+// wave meets few distinct roots.  Vertices that joined the giant are dropped as they are met.
 constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStWords = 8;
 
-// i-th vertex of this round's walk: the active list, or every vertex.
-__device__ inline uint32_t walk_count(const uint32_t* st, uint32_t round, uint32_t n, bool act) {
-  return act ? st[kStSeg + ((round - 1) & 1)] : n;
+__device__ inline uint32_t walk_count(const uint32_t* st, uint32_t round, uint32_t n) {
+  return round >= 2 ? st[kStSeg + ((round - 1) & 1)] : n;
+}
+__device__ inline uint32_t walk_vertex(const uint32_t* act_list, uint32_t round, uint32_t i) {
+  return round >= 2 ? act_list[i] : i;
 }
 
 __device__ inline bool apply_sample(const DsuDev& d, uint64_t s) {
@@ -684,10 +673,10 @@ k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ 
   if (prev_mod && !ld_acq(prev_mod)) return;
   const bool act = st && st[kStActMode] != 0;
   const uint32_t gr = act ? st[kStGiantRound] : kNoRoot;
-  const uint32_t count = st ? walk_count(st, round, d.n, act) : d.n;
+  const uint32_t count = st ? walk_count(st, round, d.n) : d.n;
   bool modified = false;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    const uint32_t v = act ? act_list[i] : i;
+    const uint32_t v = walk_vertex(act_list, round, i);
     if (root && root[v] != v) continue;
     if (v == gr) continue;  // handled once below
     if (apply_sample(d, samp[v])) modified = true;
@@ -739,14 +728,16 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
   __syncthreads();
   const uint32_t lane = lane_id();
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n, act);
-  const uint32_t v = in_walk ? (act ? act_list[i] : i) : 0;
+  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
   uint32_t r = 0;
   if (in_walk) {
-    r = dsu_find_jump(parent, v);  // leaves parent[v] == r: the walked vertices end fully compressed
+    r = dsu_find_stable(parent, v);
     root[v] = r;
   }
   __syncthreads();
+  // full path compression: every walked vertex now points at its root (no links in this kernel)
+  if (in_walk) parent[v] = r;
   const bool active = in_walk && r != gcur;  // members of the giant are not counted
   uint64_t todo = __ballot(active);
   while (todo) {
@@ -774,8 +765,8 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
   const bool act = st[kStActMode] != 0;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n, act);
-  const uint32_t v = in_walk ? (act ? act_list[i] : i) : 0;
+  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
   uint32_t c = in_walk ? cnt[v] : 0;
   if (c) cnt[v] = 0;
   if (!act && giant_ok && 2ull * c > n) {
@@ -811,13 +802,12 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
     t_cnt[k] = 0;
   }
   __syncthreads();
-  const bool act = st[kStActMode] != 0;
   const uint32_t gr = st[kStGiantRound];
   const uint32_t lane = lane_id();
   const uint64_t lt = (1ull << lane) - 1;
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n, act);
-  const uint32_t v = in_walk ? (act ? act_list[i] : i) : 0;
+  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
   const uint32_t r = in_walk ? root[v] : 0;
   const bool active = in_walk && r != gr;  // the giant supernode's members are not regrouped
   uint32_t my_slot = 0, my_pos = 0;
