@@ -159,6 +159,7 @@ _SIGS = {
     "gz_alg_digest": (ctypes.c_int, [_vp, _u64p]),
     "gz_alg_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_alg_set_query_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_alg_last_query_round_sizes": (_sz, [_vp, _vp, _vp, _sz]),
     "gz_alg_get_profile": (ctypes.c_int, [_vp, ctypes.POINTER(AlgProfile)]),
     # include/gz_serving.h
     "gz_alg_serve_trace": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _sz, _vp, _vp, ctypes.POINTER(ServeStats)]),
@@ -528,6 +529,14 @@ class CCSketchAlg:
 
     def set_profiling(self, enable: bool = True):
         _check(lib().gz_alg_set_profiling(self._h, 1 if enable else 0))
+
+    def last_query_round_sizes(self):
+        """[(vertices regrouped, giant root or None)] for rounds 1.. of the last Borůvka query."""
+        e = np.zeros(64, dtype=np.uint32)
+        g = np.zeros(64, dtype=np.uint32)
+        k = int(lib().gz_alg_last_query_round_sizes(self._h, e.ctypes.data_as(ctypes.c_void_p),
+                                                     g.ctypes.data_as(ctypes.c_void_p), 64))
+        return [(int(e[i]), None if g[i] == 0xFFFFFFFF else int(g[i])) for i in range(k)]
 
     def set_query_graphs(self, enable: bool = True):
         """Replay each query's first Borůvka rounds from a HIP graph (default) or launch them one by one."""

@@ -646,6 +646,7 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot,
 // previous round's roots, so the members of a new supernode mostly sit next to each other and a
 // wave meets few distinct roots.  Vertices that joined the giant are dropped as they are met.
 constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStWords = 8;
+constexpr uint32_t kMaxStatRounds = 64;
 
 __device__ inline uint32_t walk_count(const uint32_t* st, uint32_t round, uint32_t n) {
   return round >= 2 ? st[kStSeg + ((round - 1) & 1)] : n;
@@ -873,12 +874,16 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
             const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g,
             uint32_t round, const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
             uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
-            uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod) {
+            uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats) {
   if (!ld_acq(prev_mod)) return;
   const uint32_t lane = lane_id();
   const uint32_t giant = st[kStGiantRound];
   const bool has_giant = giant != kNoRoot;
   const uint32_t e1 = st[kStSeg + (round & 1)];
+  if (FIRST && rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
+    rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
+    rstats[2 * round + 1] = giant;
+  }
   uint32_t n_entries;
   if (!level_entries(e1, has_giant, level, &n_entries)) return;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -1287,17 +1292,21 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
     if (l == 0) {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
+                           rb.rstats);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
+                           rb.rstats);
     } else {
       if (gbytes == 4)
         hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
+                           rb.rstats);
       else
         hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev);
+                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
+                           rb.rstats);
     }
   }
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,

@@ -83,6 +83,7 @@ struct RoundBufs {
   uint64_t* part[2];     // partial-XOR entries of the multi-level reduce, 2*ceil(n/64)+2 entries of 128 u64
   uint32_t* proot[2];    // root of each partial entry
   uint64_t* samp;        // n      sample record per supernode root
+  uint32_t* rstats;      // 2 * 64 per round: vertices regrouped, giant root (diagnostics)
 };
 // Writes rb.samp: a sample record per vertex.
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,

@@ -225,12 +225,14 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
   r_members[0].alloc(n); r_members[1].alloc(n);
   r_totals.alloc(8);
+  r_stats.alloc(128);
+  launch_fill_u32(r_stats.p, 0, 128, st);
   for (int i = 0; i < 2; ++i) {
     r_part[i].alloc((size_t)(2 * waves1 + 2) * 128);
     r_proot[i].alloc(2 * waves1 + 2);
   }
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
-                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p};
+                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p, r_stats.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
@@ -1129,6 +1131,23 @@ int gz_alg_set_profiling(gz_alg* a, int enable) {
     A.prof = enable != 0;
     A.profile = gz_alg_profile{};
   });
+}
+size_t gz_alg_last_query_round_sizes(gz_alg* a, uint32_t* entries, uint32_t* giant_root, size_t cap) {
+  size_t k = 0;
+  const int rc = guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    GzAlg& A = *a->a;
+    std::lock_guard<std::mutex> lk(A.query_mtx);
+    uint32_t h[128];
+    GZ_HIP(hipMemcpyAsync(h, A.r_stats.p, sizeof(h), hipMemcpyDeviceToHost, A.st));
+    GZ_HIP(hipStreamSynchronize(A.st));
+    const size_t rounds = std::min<size_t>(A.last_rounds, 63);
+    for (size_t r = 1; r <= rounds && k < cap; ++r, ++k) {
+      if (entries) entries[k] = h[2 * r];
+      if (giant_root) giant_root[k] = h[2 * r + 1];
+    }
+  });
+  return rc == 0 ? k : 0;
 }
 int gz_alg_set_query_graphs(gz_alg* a, int enable) {
   return guarded([&] {

@@ -142,7 +142,7 @@ class GzAlg {
   DevBuf<uint8_t> qout;
   DsuDev dsu{};
   // Borůvka round buffers
-  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_totals, mod;
+  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_totals, r_stats, mod;
   DevBuf<uint64_t> r_part[2], r_samp;
   DevBuf<uint32_t> r_proot[2];
   RoundBufs rb{};

@@ -171,6 +171,10 @@ int gz_alg_get_profile(gz_alg* a, gz_alg_profile* out);
  * replays DSU reset, round 0 and the first speculative Borůvka rounds from a HIP graph built on
  * the instance's first query; 0: the same kernels are launched one by one.  Results are identical. */
 int gz_alg_set_query_graphs(gz_alg* a, int enable);
+/* No counterpart in the reference (diagnostics).  For each Borůvka round 1..last_query_rounds of
+ * the last query that ran Borůvka: the vertices regrouped (those outside a giant supernode) and
+ * the giant's root (0xFFFFFFFF when none).  Returns the number of rounds written (<= cap). */
+size_t gz_alg_last_query_round_sizes(gz_alg* a, uint32_t* entries, uint32_t* giant_root, size_t cap);
 /* Device footprint in bytes of an instance with these parameters (sketch slab + work buffers). */
 uint64_t gz_alg_footprint_bytes(uint32_t num_vertices, const gz_alg_config* cfg);
 

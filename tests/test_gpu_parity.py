@@ -330,3 +330,20 @@ def test_query_graph_replays_equal_direct_launches():
         c, lab = oracle.exact_cc(900, pairs[:i + step])
         assert cg.size() == c and np.array_equal(cg.canonical_labels(), lab)
         assert g.point_query(0, 1) == d.point_query(0, 1)
+
+
+def test_round_size_diagnostics():
+    # per-round sizes of the last Borůvka query: one entry per round, at most n vertices regrouped,
+    # and once a giant (over half of the vertices) is elected every later round regroups at most
+    # the vertices outside it
+    pairs, _ = streams.dynamic_erdos_stream(1500, 0.0012, 0.3, 0.2, 3, 21)
+    alg = gz.CCSketchAlg(1500, 77, gz.CCAlgConfiguration(eager_dsu=False))
+    alg.ingest_pairs(pairs)
+    alg.connected_components()
+    sizes = alg.last_query_round_sizes()
+    assert len(sizes) == alg.last_query_rounds
+    assert all(0 <= e <= 1500 for e, _ in sizes)
+    elected = [i for i, (_, g) in enumerate(sizes) if g is not None]
+    assert elected, sizes  # the largest component holds 1431 of 1500 vertices
+    first = elected[0]
+    assert all(g is not None and e < 750 for e, g in sizes[first:])
