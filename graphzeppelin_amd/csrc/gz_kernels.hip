@@ -645,7 +645,12 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot,
 // r - 1: every vertex outside the giant (all of them before one is elected), grouped by the
 // previous round's roots, so the members of a new supernode mostly sit next to each other and a
 // wave meets few distinct roots.  Vertices that joined the giant are dropped as they are met.
-constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStWords = 8;
+//   kStBest (u64)     before any giant: max over this round's roots holding >= n/kCandidateDiv
+//                     vertices of (count << 32 | root); the next round excludes that supernode.
+//                     The complement sum needs no majority -- any supernode's sum is the XOR of the
+//                     vertices outside it -- so a large one can be left out a round early.
+constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStBest = 6, kStWords = 8;
+constexpr uint32_t kCandidateDiv = 32;
 constexpr uint32_t kMaxStatRounds = 64;
 
 __device__ inline uint32_t walk_count(const uint32_t* st, uint32_t round, uint32_t n) {
@@ -718,12 +723,17 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
     t_key[i] = kEmptyKey;
     t_cnt[i] = 0;
   }
-  const uint32_t gfirst = st[kStGiantFirst];
+  uint32_t gfirst = st[kStGiantFirst];
+  if (gfirst == kNoRoot) {  // a candidate recorded by the previous round (0 when none)
+    const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
+    if (best) gfirst = (uint32_t)best;
+  }
   const bool act = gfirst != kNoRoot;
   const uint32_t gcur = act ? dsu_find_stable(parent, gfirst) : kNoRoot;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[kStSeg + (round & 1)] = 0;
     st[kStGiantRound] = gcur;
+    st[kStGiantFirst] = gfirst;
     st[kStActMode] = act ? 1u : 0u;
   }
   __syncthreads();
@@ -774,6 +784,8 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
     st[kStGiantFirst] = v;
     st[kStGiantRound] = v;
     c = 0;
+  } else if (!act && giant_ok && c >= 2 && c >= n / kCandidateDiv) {  // at most kCandidateDiv roots
+    atomicMax(reinterpret_cast<unsigned long long*>(st + kStBest), ((unsigned long long)c << 32) | v);
   }
   const uint64_t incl = wave_incl_scan_u64(c);
   if (lane == kWave - 1) wsum[wid] = incl;
@@ -952,7 +964,7 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
 
   uint64_t acc_a = 0, acc_g = 0, all_a = 0, all_g = 0;
   uint32_t cur = first_root, seg_start = p0;
-  constexpr uint32_t U = FIRST ? 8 : 16;  // loads in flight per lane
+  constexpr uint32_t U = 16;  // loads in flight per lane
   for (uint32_t j = 0; j < len; j += U) {
     uint64_t va[U], vg[U];
 #pragma unroll
