@@ -1257,9 +1257,10 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
 }
 
 // Host bound on each level's entry count (the kernels compute the exact counts on the device):
-// the larger of the chain without a giant (e1 <= n) and with one (e1 < n/2, level 2 = 3 * W1).
+// the larger of the chain without a supernode summed from its complement (level 2 = 2 * W1) and
+// with one (level 2 = 3 * W1); e1 <= n in both.
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) {
-  uint32_t a = n, b = n / 2;
+  uint32_t a = n, b = n;
   size_t L = 0;
   bool a_done = false, b_done = false;
   while (true) {

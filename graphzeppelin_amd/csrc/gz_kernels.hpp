@@ -80,7 +80,7 @@ struct RoundBufs {
   uint32_t* cursor;      // n      scatter cursor per root
   uint32_t* members[2];  // n each vertices grouped by root, by round parity (also the next active list)
   uint32_t* totals;      // 8      round state words (entry counts, giant roots, walk mode)
-  uint64_t* part[2];     // partial-XOR entries of the multi-level reduce, 2*ceil(n/64)+2 entries of 128 u64
+  uint64_t* part[2];     // partial-XOR entries of the multi-level reduce, 3*ceil(n/64)+2 entries of 128 u64
   uint32_t* proot[2];    // root of each partial entry
   uint64_t* samp;        // n      sample record per supernode root
   uint32_t* rstats;      // 2 * 64 per round: vertices regrouped, giant root (diagnostics)

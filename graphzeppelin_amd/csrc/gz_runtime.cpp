@@ -170,8 +170,9 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
   // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
-  // forest edge (8 each), partial entries (2 x 1 KiB per 32 vertices), spanning-forest hash set (16)
-  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 64 + 16) + 4096;
+  // forest edge (8 each), partial entries (2 buffers x 3 x 1 KiB per 64 vertices), spanning-forest
+  // hash set (16)
+  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 96 + 16) + 8192;
   return slab + ingest + rounds;
 }
 
@@ -228,8 +229,10 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   r_stats.alloc(128);
   launch_fill_u32(r_stats.p, 0, 128, st);
   for (int i = 0; i < 2; ++i) {
-    r_part[i].alloc((size_t)(2 * waves1 + 2) * 128);
-    r_proot[i].alloc(2 * waves1 + 2);
+    // level 1 writes head and tail partials per wave, plus one total per wave when a supernode
+    // is summed from its complement (which may hold as few as n / 32 vertices): 3 per wave
+    r_part[i].alloc((size_t)(3 * waves1 + 2) * 128);
+    r_proot[i].alloc(3 * waves1 + 2);
   }
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
                  {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p, r_stats.p};
