@@ -649,7 +649,9 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot,
 //                     vertices of (count << 32 | root); the next round excludes that supernode.
 //                     The complement sum needs no majority -- any supernode's sum is the XOR of the
 //                     vertices outside it -- so a large one can be left out a round early.
-constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStBest = 6, kStWords = 8;
+//   kStDone           workgroups of this round's upper-level reduce that have finished level 2
+constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStDone = 5, kStBest = 6;
+constexpr uint32_t kStWords = 8;
 constexpr uint32_t kCandidateDiv = 32;
 constexpr uint32_t kMaxStatRounds = 64;
 
@@ -735,6 +737,7 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
     st[kStGiantRound] = gcur;
     st[kStGiantFirst] = gfirst;
     st[kStActMode] = act ? 1u : 0u;
+    st[kStDone] = 0;
   }
   __syncthreads();
   const uint32_t lane = lane_id();
@@ -880,29 +883,16 @@ __device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, ui
   return true;
 }
 
+// One wave's part of one level: entries [64*wave, 64*wave + 64) of n_entries.
 template <class G, bool FIRST>
-__global__ void __launch_bounds__(256)
-k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
-            const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g,
-            uint32_t round, const uint64_t* __restrict__ in_part, const uint32_t* __restrict__ in_root,
-            uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
-            uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats) {
-  if (!ld_acq(prev_mod)) return;
+__device__ inline void reduce_wave(uint32_t wave, uint32_t n_entries, uint32_t giant, const uint32_t* __restrict__ members,
+                                   const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab,
+                                   const Geo& g, uint32_t round, const uint64_t* __restrict__ in_part,
+                                   const uint32_t* __restrict__ in_root, uint64_t* __restrict__ out_part,
+                                   uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
+                                   uint32_t* __restrict__ flags) {
   const uint32_t lane = lane_id();
-  const uint32_t giant = st[kStGiantRound];
   const bool has_giant = giant != kNoRoot;
-  const uint32_t e1 = st[kStSeg + (round & 1)];
-  if (FIRST && rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
-    rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
-    rstats[2 * round + 1] = giant;
-  }
-  uint32_t n_entries;
-  if (!level_entries(e1, has_giant, level, &n_entries)) return;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  if (FIRST && has_giant && e1 == 0 && wave == 0) {  // every vertex is in the giant: its sum is zero
-    if (lane == 0) samp[giant] = kSampZero;
-    return;
-  }
   const uint32_t p0 = wave * kWave;
   if (p0 >= n_entries) return;
   const uint32_t p1 = (p0 + kWave < n_entries) ? p0 + kWave : n_entries;
@@ -964,7 +954,7 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
 
   uint64_t acc_a = 0, acc_g = 0, all_a = 0, all_g = 0;
   uint32_t cur = first_root, seg_start = p0;
-  constexpr uint32_t U = 16;  // loads in flight per lane
+  constexpr uint32_t U = FIRST ? 8 : 16;  // loads in flight per lane
   for (uint32_t j = 0; j < len; j += U) {
     uint64_t va[U], vg[U];
 #pragma unroll
@@ -1017,6 +1007,73 @@ k_bv_reduce(uint32_t level, const uint32_t* __restrict__ st, const uint32_t* __r
   // unused output slots carry no root (and their values are never read)
   if (lane == 0 && !wrote_head) out_root[2 * wave] = kNoRoot;
   if (lane == 0 && !wrote_tail) out_root[2 * wave + 1] = kNoRoot;
+}
+
+
+template <class G>
+__global__ void __launch_bounds__(256)
+k_bv_reduce_first(const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
+                  const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
+                  uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
+                  uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats) {
+  if (!ld_acq(prev_mod)) return;
+  const uint32_t giant = st[kStGiantRound];
+  const uint32_t e1 = st[kStSeg + (round & 1)];
+  if (rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
+    rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
+    rstats[2 * round + 1] = giant;
+  }
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (giant != kNoRoot && e1 == 0) {  // every vertex is in the giant: its sum is zero
+    if (wave == 0 && lane_id() == 0) samp[giant] = kSampZero;
+    return;
+  }
+  reduce_wave<G, true>(wave, e1, giant, members, vroot, slab, g, round, nullptr, nullptr, out_part, out_root, samp,
+                       flags);
+}
+
+// Levels >= 2 of the reduce.  Every workgroup takes its waves of level `level`.  With `tail`
+// (the host sets it once the following levels hold at most kTailEntries entries), the last
+// workgroup to finish -- a counter per round with device-scope fences around it, so no workgroup
+// ever waits -- then runs the remaining levels alone, so a round needs one kernel for its small
+// upper levels.  Level l reads part[l & 1] and writes part[(l - 1) & 1].
+constexpr uint32_t kTailEntries = 1024;
+template <class G>
+__global__ void __launch_bounds__(256)
+k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const Geo g, uint32_t round,
+               uint64_t* __restrict__ part0, uint64_t* __restrict__ part1, uint32_t* __restrict__ proot0,
+               uint32_t* __restrict__ proot1, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
+               const uint32_t* __restrict__ prev_mod) {
+  if (!ld_acq(prev_mod)) return;
+  const uint32_t giant = st[kStGiantRound];
+  const bool has_giant = giant != kNoRoot;
+  const uint32_t e1 = st[kStSeg + (round & 1)];
+  if (has_giant && e1 == 0) return;
+  uint32_t ne;
+  if (!level_entries(e1, has_giant, level0, &ne)) return;  // uniform over the grid: an earlier level was the last
+  uint64_t* part[2] = {part0, part1};
+  uint32_t* proot[2] = {proot0, proot1};
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  reduce_wave<G, false>(wave, ne, giant, nullptr, nullptr, nullptr, g, round, part[level0 & 1], proot[level0 & 1],
+                        part[(level0 - 1) & 1], proot[(level0 - 1) & 1], samp, flags);
+  if (!tail) return;
+  __shared__ uint32_t s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(st + kStDone, 1u) == gridDim.x - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const uint32_t wid = threadIdx.x / kWave, nwb = blockDim.x / kWave;
+  for (uint32_t level = level0 + 1;; ++level) {
+    if (!level_entries(e1, has_giant, level, &ne)) break;
+    const uint32_t nw = (ne + kWave - 1) / kWave;
+    for (uint32_t w = wid; w < nw; w += nwb)
+      reduce_wave<G, false>(w, ne, giant, nullptr, nullptr, nullptr, g, round, part[level & 1], proot[level & 1],
+                            part[(level - 1) & 1], proot[(level - 1) & 1], samp, flags);
+    __threadfence_block();
+    __syncthreads();
+  }
 }
 
 // ConnectedComponents (src/return_types.cpp:5-19): root of every vertex and the root count.
@@ -1293,34 +1350,24 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
                      act_list, prev);
   uint32_t ent[32];
   const size_t L = round_levels(n, ent, 32);
-  for (size_t l = 0; l < L; ++l) {
-    const uint32_t e = ent[l];
-    const uint32_t waves = (e + kWave - 1) / kWave;
-    const uint32_t blocks = std::max(1u, (waves * kWave + 255) / 256);
+  const uint32_t blocks1 = std::max(1u, (ent[0] + 255) / 256);  // one wave per 64 entries
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_bv_reduce_first<uint32_t>, dim3(blocks1), dim3(256), 0, s, st, members, rb.root, slab, g,
+                       round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
+  else
+    hipLaunchKernelGGL(k_bv_reduce_first<uint64_t>, dim3(blocks1), dim3(256), 0, s, st, members, rb.root, slab, g,
+                       round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
+  for (size_t l = 1; l < L; ++l) {  // level l + 1
+    const uint32_t blocks = std::max(1u, (ent[l] + 255) / 256);
+    const bool tail = l + 1 >= L || ent[l + 1] <= kTailEntries;
     const uint32_t level = (uint32_t)l + 1;
-    uint64_t* in_p = rb.part[(l + 1) & 1];
-    uint32_t* in_r = rb.proot[(l + 1) & 1];
-    uint64_t* out_p = rb.part[l & 1];
-    uint32_t* out_r = rb.proot[l & 1];
-    if (l == 0) {
-      if (gbytes == 4)
-        hipLaunchKernelGGL((k_bv_reduce<uint32_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
-                           rb.rstats);
-      else
-        hipLaunchKernelGGL((k_bv_reduce<uint64_t, true>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
-                           rb.rstats);
-    } else {
-      if (gbytes == 4)
-        hipLaunchKernelGGL((k_bv_reduce<uint32_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
-                           rb.rstats);
-      else
-        hipLaunchKernelGGL((k_bv_reduce<uint64_t, false>), dim3(blocks), dim3(256), 0, s, level, st, members, rb.root,
-                           slab, g, round, in_p, in_r, out_p, out_r, rb.samp, d.flags, prev,
-                           rb.rstats);
-    }
+    if (gbytes == 4)
+      hipLaunchKernelGGL(k_bv_reduce_up<uint32_t>, dim3(blocks), dim3(256), 0, s, level, tail ? 1u : 0u, st, g, round,
+                         rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
+    else
+      hipLaunchKernelGGL(k_bv_reduce_up<uint64_t>, dim3(blocks), dim3(256), 0, s, level, tail ? 1u : 0u, st, g, round,
+                         rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
+    if (tail) break;
   }
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
                      act_list, prev, cur);
