@@ -296,8 +296,20 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   const uint32_t di = kWave / C, dc = kWave % C;
   __syncthreads();
 
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const Tile tl = tiles[t];
+  // The next tile's descriptor and each lane's first index are loaded before the current tile's
+  // write-back, so those two dependent round trips overlap it instead of opening the next tile.
+  auto first_idx = [&](const Tile& tt) -> uint64_t {
+    if (i0 >= tt.count) return 0;
+    return RAW ? raw_idx[tt.begin + i0] : concat_pairing(tt.vertex, dsts[tt.begin + i0]);
+  };
+  uint32_t t = blockIdx.x;
+  Tile tl{0, 0, 0, 0};
+  uint64_t idx0 = 0;
+  if (t < ntiles) {
+    tl = tiles[t];
+    idx0 = first_idx(tl);
+  }
+  while (t < ntiles) {
     unsigned char* row = slab + (size_t)tl.vertex * g.row;
 
     uint64_t det_a = 0;
@@ -305,7 +317,8 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     const uint32_t npairs = tl.count * C;
     uint32_t i = i0, c = c0;
     for (uint32_t pr = lane; pr < npairs; pr += kWave) {
-      const uint64_t idx = RAW ? raw_idx[tl.begin + i] : concat_pairing(tl.vertex, dsts[tl.begin + i]);
+      const uint64_t idx = pr == lane ? idx0
+                                      : (RAW ? raw_idx[tl.begin + i] : concat_pairing(tl.vertex, dsts[tl.begin + i]));
       const uint64_t pre = xxh3_prep(idx);
       const G chk = (G)xxh3_mix(pre, g.chk_bf);
       if (c == 0) {
@@ -337,6 +350,14 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
       s_g[det_slot] ^= det_g;
     }
     __syncthreads();
+
+    const uint32_t tn = t + gridDim.x;
+    Tile tl_next{0, 0, 0, 0};
+    uint64_t idx0_next = 0;
+    if (tn < ntiles) {
+      tl_next = tiles[tn];
+      idx0_next = first_idx(tl_next);
+    }
 
     // write-back: read-xor-write of the touched records, four LDS entries per lane in flight
     for (uint32_t e0 = 0; e0 < lds_n; e0 += 4 * kWave) {
@@ -378,6 +399,9 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
       }
     }
     __syncthreads();
+    t = tn;
+    tl = tl_next;
+    idx0 = idx0_next;
   }
 }
 
