@@ -54,7 +54,14 @@ struct GroupBufs {
   uint32_t* dst_arr;   // 2 * max_pairs
   Tile* tiles;         // max tiles
   uint32_t* bad;       // set to 1 when a pair names a vertex >= n
+  // bucket partition (used when non-null and the vertex count allows it)
+  uint2* rec;          // 2 * max_pairs records (vertex, other end)
+  uint32_t* hist;      // part_hist_len()
+  uint32_t* hoff;      // part_hist_len()
+  uint64_t* hpartials; // group_partials_len(part_hist_len())
+  uint32_t* htotals;   // 4
 };
+size_t part_hist_len();
 size_t group_partials_len(uint32_t n);
 // pairs: 2*count u32 (src,dst) in HBM.  Produces tiles of at most `tile` destinations.
 // Profiling: per-block accumulators of a grouped flush's algorithmic bytes (kProfSlots u64,

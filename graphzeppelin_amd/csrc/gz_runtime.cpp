@@ -155,7 +155,7 @@ void GzSketch::deserialize(const void* in, size_t nbytes) {  // src/sketch.cpp:2
 // ================================================================================================
 static size_t default_cap_pairs(uint32_t n, size_t desired) {
   const size_t want = (size_t)n * desired / 2;
-  return std::min<size_t>(size_t(1) << 26, std::max<size_t>(size_t(1) << 20, want));
+  return std::min<size_t>(size_t(1) << 25, std::max<size_t>(size_t(1) << 20, want));
 }
 
 uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
@@ -167,7 +167,8 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const size_t desired = (size_t)(nb * (8 + gbytes) / 4 * c.batch_factor);
   const uint64_t cap = c.flush_threshold ? c.flush_threshold : default_cap_pairs(n, desired);
   const uint32_t t = c.tile ? c.tile : 1024;
-  const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ +
+  const uint64_t part = n <= (1u << 24) ? cap * 16 /*records*/ + 8 * 1048577ull + (1 << 16) : 0;
+  const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ + part +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
   // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
   // forest edge (8 each), partial entries (2 buffers x 3 x 1 KiB per 64 vertices), spanning-forest
@@ -248,7 +249,19 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   g_partials.alloc(group_partials_len(n));
   g_tiles.alloc(2 * cap_pairs / tile + n + 1);
   gbufs = GroupBufs{g_cnt.p, g_off.p, g_toff.p, g_cursor.p, g_partials.p, g_totals.p, dst_arr.p, g_tiles.p,
-                    flags.p + kFlagBadInput};
+                    flags.p + kFlagBadInput, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (n <= (1u << 24)) {  // bucket-partition grouping (no device-scope atomics)
+    g_rec.alloc(2 * cap_pairs);
+    g_hist.alloc(part_hist_len());
+    g_hoff.alloc(part_hist_len());
+    g_hpartials.alloc(group_partials_len((uint32_t)part_hist_len()));
+    g_htotals.alloc(4);
+    gbufs.rec = g_rec.p;
+    gbufs.hist = g_hist.p;
+    gbufs.hoff = g_hoff.p;
+    gbufs.hpartials = g_hpartials.p;
+    gbufs.htotals = g_htotals.p;
+  }
 
   // host staging
   stage_cap = std::max<size_t>(desired_updates_per_batch(), cfg.batched ? (size_t(1) << 22) : 0);

@@ -149,7 +149,9 @@ class GzAlg {
   // device stream ingest (pending pairs -> grouping -> tiles)
   size_t cap_pairs;
   size_t pending_n = 0;
-  DevBuf<uint32_t> pending, dst_arr, g_cnt, g_off, g_toff, g_cursor, g_totals;
+  DevBuf<uint32_t> pending, dst_arr, g_cnt, g_off, g_toff, g_cursor, g_totals, g_hist, g_hoff, g_htotals;
+  DevBuf<uint2> g_rec;
+  DevBuf<uint64_t> g_hpartials;
   DevBuf<uint64_t> g_partials;
   DevBuf<Tile> g_tiles;
   DevBuf<uint8_t> merged;
