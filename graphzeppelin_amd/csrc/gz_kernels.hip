@@ -269,7 +269,7 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
 constexpr uint32_t kLdsDepth = 8, kLdsPitch = 9;
 
 template <class G, bool RAW>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave, 8)  // 8 waves per SIMD: 32 tiles in flight per CU
 k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntiles_dev, uint32_t ntiles_host,
                const uint32_t* __restrict__ dsts, const uint64_t* __restrict__ raw_idx,
                unsigned char* __restrict__ slab, Geo g, const uint64_t* __restrict__ col_bf) {
@@ -296,51 +296,64 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   const uint32_t di = kWave / C, dc = kWave % C;
   __syncthreads();
 
-  // The next tile's descriptor and each lane's first index are loaded before the current tile's
-  // write-back, so those two dependent round trips overlap it instead of opening the next tile.
-  auto first_idx = [&](const Tile& tt) -> uint64_t {
-    if (i0 >= tt.count) return 0;
-    return RAW ? raw_idx[tt.begin + i0] : concat_pairing(tt.vertex, dsts[tt.begin + i0]);
+  // Updates are taken in chunks of 64: lane l loads update u0 + l once and hashes its index and
+  // checksum once; the chunk's (update, column) pairs then fetch those from the owning lane with
+  // shuffles, so each pair costs one column hash and no memory round trip.  The next tile's
+  // descriptor and each lane's first index are loaded before the current tile's write-back.
+  auto load_idx = [&](const Tile& tt, uint32_t u) -> uint64_t {
+    return RAW ? raw_idx[tt.begin + u] : concat_pairing(tt.vertex, dsts[tt.begin + u]);
   };
   uint32_t t = blockIdx.x;
   Tile tl{0, 0, 0, 0};
   uint64_t idx0 = 0;
   if (t < ntiles) {
     tl = tiles[t];
-    idx0 = first_idx(tl);
+    if (lane < tl.count) idx0 = load_idx(tl, lane);
   }
   while (t < ntiles) {
     unsigned char* row = slab + (size_t)tl.vertex * g.row;
 
     uint64_t det_a = 0;
     G det_g = 0;
-    const uint32_t npairs = tl.count * C;
-    uint32_t i = i0, c = c0;
-    for (uint32_t pr = lane; pr < npairs; pr += kWave) {
-      const uint64_t idx = pr == lane ? idx0
-                                      : (RAW ? raw_idx[tl.begin + i] : concat_pairing(tl.vertex, dsts[tl.begin + i]));
-      const uint64_t pre = xxh3_prep(idx);
-      const G chk = (G)xxh3_mix(pre, g.chk_bf);
-      if (c == 0) {
+    for (uint32_t u0 = 0; u0 < tl.count; u0 += kWave) {
+      const uint32_t m = tl.count - u0 < kWave ? tl.count - u0 : kWave;
+      uint64_t my_pre = 0;
+      G my_chk = 0;
+      if (lane < m) {
+        const uint64_t idx = u0 == 0 ? idx0 : load_idx(tl, u0 + lane);
+        my_pre = xxh3_prep(idx);
+        my_chk = (G)xxh3_mix(my_pre, g.chk_bf);
         det_a ^= idx;
-        det_g ^= chk;
+        det_g ^= my_chk;
       }
-      const uint64_t h = xxh3_mix(pre, s_bf[c]);
-      const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);  // == D: no bucket
-      if (depth < D) {
-        if (depth < kLdsDepth) {
-          const uint32_t li = c * kLdsPitch + depth;
-          lds_xor(s_a + li, idx);
-          lds_xor(s_g + li, chk);
-        } else {
-          xor_rec_atomic<G>(row + (size_t)(c * D + depth) * RB, idx, chk);
+      const uint32_t np = m * C;
+      const uint32_t iters = (np + kWave - 1) / kWave;  // uniform over the wave
+      uint32_t q = i0, c = c0;
+      for (uint32_t it = 0; it < iters; ++it) {
+        // shuffles with the whole wave active; q < 64 always
+        const uint64_t pre = shfl_u64(my_pre, (int)(q & (kWave - 1)));
+        const G chk = sizeof(G) == 4 ? (G)__shfl((unsigned)(uint32_t)my_chk, (int)(q & (kWave - 1)))
+                                     : (G)shfl_u64((uint64_t)my_chk, (int)(q & (kWave - 1)));
+        if (it * kWave + lane < np) {
+          const uint64_t idx = (pre << 32) | (pre >> 32);  // xxh3_prep swaps the halves
+          const uint64_t h = xxh3_mix(pre, s_bf[c]);
+          const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);  // == D: no bucket
+          if (depth < D) {
+            if (depth < kLdsDepth) {
+              const uint32_t li = c * kLdsPitch + depth;
+              lds_xor(s_a + li, idx);
+              lds_xor(s_g + li, chk);
+            } else {
+              xor_rec_atomic<G>(row + (size_t)(c * D + depth) * RB, idx, chk);
+            }
+          }
         }
-      }
-      i += di;
-      c += dc;
-      if (c >= C) {
-        c -= C;
-        ++i;
+        q += di;
+        c += dc;
+        if (c >= C) {
+          c -= C;
+          ++q;
+        }
       }
     }
     det_a = wave_xor_u64(det_a);
@@ -356,7 +369,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     uint64_t idx0_next = 0;
     if (tn < ntiles) {
       tl_next = tiles[tn];
-      idx0_next = first_idx(tl_next);
+      if (lane < tl_next.count) idx0_next = load_idx(tl_next, lane);
     }
 
     // write-back: read-xor-write of the touched records, four LDS entries per lane in flight
@@ -545,6 +558,105 @@ k_make_tiles(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
     uint64_t t = 0;
     for (int w = 0; w < 256 / kWave; ++w) t += s_part[w];
     acc[blockIdx.x] += t;
+  }
+}
+
+// Grouping by vertex without device-scope atomics (vertex counts up to 2^24): a bucket partition
+// over vertex ranges of R = 2^shift vertices (at most kPartBins ranges).
+//   1. k_part_hist: each workgroup takes a contiguous chunk of the pairs and counts the directed
+//      entries per range in LDS; the counts go to hist[range * nblk + block].
+//   2. an exclusive scan of hist (range-major), so each (range, block) owns a slot interval.
+//   3. k_part_scatter: the same chunks again; LDS cursors place the records (vertex, other end)
+//      in range order.
+//   4. k_part_group: one workgroup per range counts its vertices' entries in LDS, scans them, and
+//      writes cnt[v] and each vertex's destinations contiguously (vertex order, the same layout
+//      the vertex-count scan then gives).  The order of one vertex's destinations is arbitrary,
+//      which XOR does not see.
+constexpr uint32_t kPartBins = 4096, kPartBlock = 1024, kPartMaxBlocks = 256, kPartMaxRange = 4096;
+
+__device__ inline void part_chunk(size_t count, uint32_t nblk, size_t* lo, size_t* hi) {
+  const size_t per = (count + nblk - 1) / nblk;
+  *lo = (size_t)blockIdx.x * per;
+  *hi = *lo + per < count ? *lo + per : count;
+}
+
+__global__ void __launch_bounds__(kPartBlock)
+k_part_hist(const uint2* __restrict__ pairs, size_t count, uint32_t n, uint32_t shift, uint32_t bins, uint32_t nblk,
+            uint32_t* __restrict__ hist, uint32_t* __restrict__ bad_flag) {
+  __shared__ uint32_t h[kPartBins];
+  for (uint32_t b = threadIdx.x; b < bins; b += kPartBlock) h[b] = 0;
+  __syncthreads();
+  size_t lo, hi;
+  part_chunk(count, nblk, &lo, &hi);
+  for (size_t k = lo + threadIdx.x; k < hi; k += kPartBlock) {
+    const uint2 p = pairs[k];
+    if (p.x >= n || p.y >= n) {
+      st_rlx(bad_flag, 1u);
+      continue;
+    }
+    atomicAdd(h + (p.x >> shift), 1u);
+    atomicAdd(h + (p.y >> shift), 1u);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < bins; b += kPartBlock) hist[(size_t)b * nblk + blockIdx.x] = h[b];
+}
+
+__global__ void __launch_bounds__(kPartBlock)
+k_part_scatter(const uint2* __restrict__ pairs, size_t count, uint32_t n, uint32_t shift, uint32_t bins,
+               uint32_t nblk, const uint32_t* __restrict__ hoff, uint2* __restrict__ rec) {
+  __shared__ uint32_t cur[kPartBins];
+  for (uint32_t b = threadIdx.x; b < bins; b += kPartBlock) cur[b] = hoff[(size_t)b * nblk + blockIdx.x];
+  __syncthreads();
+  size_t lo, hi;
+  part_chunk(count, nblk, &lo, &hi);
+  for (size_t k = lo + threadIdx.x; k < hi; k += kPartBlock) {
+    const uint2 p = pairs[k];
+    if (p.x >= n || p.y >= n) continue;
+    rec[atomicAdd(cur + (p.x >> shift), 1u)] = make_uint2(p.x, p.y);
+    rec[atomicAdd(cur + (p.y >> shift), 1u)] = make_uint2(p.y, p.x);
+  }
+}
+
+__global__ void __launch_bounds__(kPartBlock)
+k_part_group(const uint2* __restrict__ rec, uint32_t n, uint32_t shift, uint32_t bins, uint32_t nblk,
+             const uint32_t* __restrict__ hoff, uint32_t* __restrict__ cnt, uint32_t* __restrict__ dst_arr) {
+  __shared__ uint32_t lc[kPartMaxRange];
+  __shared__ uint32_t wsum[kPartBlock / kWave];
+  const uint32_t b = blockIdx.x;
+  const uint32_t base = b << shift, R = 1u << shift;
+  const uint32_t nv = n - base < R ? n - base : R;
+  const uint32_t s = hoff[(size_t)b * nblk];
+  const uint32_t e = hoff[(size_t)(b + 1) * nblk];  // hoff[bins * nblk] holds the total
+  for (uint32_t k = threadIdx.x; k < R; k += kPartBlock) lc[k] = 0;
+  __syncthreads();
+  for (uint32_t j = s + threadIdx.x; j < e; j += kPartBlock) atomicAdd(lc + (rec[j].x - base), 1u);
+  __syncthreads();
+  // exclusive scan of lc[0..R): each thread owns a run of R / kPartBlock (>= 1) consecutive slots
+  const uint32_t per = (R + kPartBlock - 1) / kPartBlock;
+  const uint32_t k0 = threadIdx.x * per;
+  uint32_t mine = 0;
+  for (uint32_t k = 0; k < per; ++k)
+    if (k0 + k < R) {
+      const uint32_t c = lc[k0 + k];
+      if (k0 + k < nv) cnt[base + k0 + k] = c;
+      mine += c;
+    }
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  const uint64_t incl = wave_incl_scan_u64(mine);
+  if (lane == kWave - 1) wsum[wid] = (uint32_t)incl;
+  __syncthreads();
+  uint32_t run = (uint32_t)incl - mine;
+  for (uint32_t w = 0; w < wid; ++w) run += wsum[w];
+  for (uint32_t k = 0; k < per; ++k)
+    if (k0 + k < R) {
+      const uint32_t c = lc[k0 + k];
+      lc[k0 + k] = s + run;  // becomes the vertex's cursor
+      run += c;
+    }
+  __syncthreads();
+  for (uint32_t j = s + threadIdx.x; j < e; j += kPartBlock) {
+    const uint2 r = rec[j];
+    dst_arr[atomicAdd(lc + (r.x - base), 1u)] = r.y;
   }
 }
 
@@ -1301,14 +1413,37 @@ size_t group_partials_len(uint32_t n) { return (n + kScanSpan - 1) / kScanSpan +
 
 void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile, const GroupBufs& b, hipStream_t s,
                   const FlushBytes* fb) {
-  launch_fill_u32(b.cnt, 0, n, s);
   const uint2* p2 = reinterpret_cast<const uint2*>(pairs);
-  if (count) hipLaunchKernelGGL(k_count_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cnt, b.bad);
-  scan_counts(b.cnt, n, tile, b.partials, b.off, b.toff, b.cursor, b.totals, s);
-  if (count) hipLaunchKernelGGL(k_scatter_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cursor, b.dst_arr);
+  uint32_t shift = 5;
+  while ((size_t(1) << shift) * kPartBins < n) ++shift;
+  if (b.rec && (1u << shift) <= kPartMaxRange) {
+    const uint32_t bins = (uint32_t)((n + (1u << shift) - 1) >> shift);
+    const uint32_t nblk = (uint32_t)std::max<size_t>(1, std::min<size_t>(kPartMaxBlocks, (count + 4095) / 4096));
+    const uint32_t hn = bins * nblk;
+    if (count) {
+      hipLaunchKernelGGL(k_part_hist, dim3(nblk), dim3(kPartBlock), 0, s, p2, count, n, shift, bins, nblk, b.hist,
+                         b.bad);
+    } else {
+      launch_fill_u32(b.hist, 0, hn, s);
+    }
+    scan_counts(b.hist, hn, 0, b.hpartials, b.hoff, nullptr, nullptr, b.htotals, s);
+    if (count)
+      hipLaunchKernelGGL(k_part_scatter, dim3(nblk), dim3(kPartBlock), 0, s, p2, count, n, shift, bins, nblk, b.hoff,
+                         b.rec);
+    hipLaunchKernelGGL(k_part_group, dim3(bins), dim3(kPartBlock), 0, s, b.rec, n, shift, bins, nblk, b.hoff, b.cnt,
+                       b.dst_arr);
+    scan_counts(b.cnt, n, tile, b.partials, b.off, b.toff, nullptr, b.totals, s);
+  } else {
+    launch_fill_u32(b.cnt, 0, n, s);
+    if (count) hipLaunchKernelGGL(k_count_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cnt, b.bad);
+    scan_counts(b.cnt, n, tile, b.partials, b.off, b.toff, b.cursor, b.totals, s);
+    if (count) hipLaunchKernelGGL(k_scatter_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cursor, b.dst_arr);
+  }
   hipLaunchKernelGGL(k_make_tiles, dim3(grid_for(n, 256, kProfSlots)), dim3(256), 0, s, b.cnt, b.off, b.toff, n, tile,
                      b.tiles, fb ? fb->acc : nullptr, fb ? fb->S : 0, fb ? fb->per_update : 0);
 }
+
+size_t part_hist_len() { return (size_t)kPartBins * kPartMaxBlocks + 1; }
 
 void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d, hipStream_t s) {
   if (!count) return;
