@@ -989,53 +989,60 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
 }
 
 // Segmented XOR-reduce + sample, one level (range_merge of column `round` + the deterministic
-// bucket over every member of a supernode, then sample_supernode).  Entries are grouped by root;
-// a wave owns 64 consecutive entries, lane l < D accumulates depth l of the column and lane D the
-// deterministic bucket.  A supernode whose entries all lie inside the wave is sampled from
+// bucket over every member of a supernode, then sample_supernode).  Entries are grouped by root.
+// A wave takes 64 consecutive entries; lane l < D accumulates depth l of the column and lane D the
+// deterministic bucket.  A supernode whose entries all lie inside the range is sampled from
 // registers; its sample record goes to samp[root] for k_apply_samples.  A supernode that crosses
-// the wave's first (head) or last (tail) boundary leaves its partial XOR in out[2w] / out[2w+1]
-// under its root; unused outputs get root kNoRoot and zeros.  The next level reduces those entries
-// the same way, so a supernode spanning many waves is combined without any atomics; the last level
-// has a single wave.
+// the range's first (head) or last (tail) boundary leaves its partial XOR in the range's head or
+// tail slot under its root; a slot left unused gets root kNoRoot (its value is never read).
+//
+// A workgroup of 16 waves covers 1024 entries: the waves leave their slots in LDS, then one wave
+// reduces those 32 slots the same way, so a workgroup emits at most two partials (plus one
+// total, below) per 1024 entries.  The next level reduces the partials of all workgroups, so a
+// supernode of any size is combined without atomics, and at n = 2^18 the second level fits in one
+// workgroup.
 //
 // Giant supernode.  Every edge update XORs the same buckets into both endpoints' sketches, so the
 // XOR of all vertex sketches is zero and a supernode's sum equals the XOR of every vertex outside
-// it.  When one root holds more than half of the vertices (and the sketches are known to sum to
-// zero) its members are not regrouped: each level-1 wave also XORs all of its entries into a
-// total, the totals form one extra segment [2W, 3W) under the giant's root at level 2, and the
-// giant is sampled from that sum -- the same bytes the member-wise merge would give.
+// it.  When a supernode is summed from its complement (the sketches known to sum to zero) its
+// members are not regrouped: each level-1 workgroup also XORs all of its entries into a total,
+// the totals form one extra segment [2B, 3B) under the giant's root at level 2, and the giant is
+// sampled from that sum -- the same bytes the member-wise merge would give.
 //
 // Entry counts are on the device: level 1 has e1 = (entries regrouped) entries; level 2 has
-// 2*W1 (+ W1 totals with a giant) with W1 = ceil(e1/64); later levels 2*ceil(e/64).  A level whose
-// predecessor had a single wave is a no-op.
+// 2*B1 (+ B1 totals with a giant) with B1 = ceil(e1/1024); later levels 2*ceil(e/1024).  A level
+// whose predecessor had a single workgroup is a no-op.
+constexpr uint32_t kRedWaves = 16, kRedBlock = kRedWaves * kWave, kRedSpan = kRedWaves * kWave;
+
 __device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, uint32_t* e) {
   uint32_t cur = e1;
   for (uint32_t l = 1; l < level; ++l) {
-    const uint32_t w = (cur + kWave - 1) / kWave;
-    if (w <= 1) return false;
-    cur = 2 * w + ((l == 1 && giant) ? w : 0);
+    const uint32_t b = (cur + kRedSpan - 1) / kRedSpan;
+    if (b <= 1) return false;
+    cur = 2 * b + ((l == 1 && giant) ? b : 0);
   }
   *e = cur;
   return true;
 }
 
-// One wave's part of one level: entries [64*wave, 64*wave + 64) of n_entries.
-template <class G, bool FIRST>
-__device__ inline void reduce_wave(uint32_t wave, uint32_t n_entries, uint32_t giant, const uint32_t* __restrict__ members,
-                                   const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab,
-                                   const Geo& g, uint32_t round, const uint64_t* __restrict__ in_part,
-                                   const uint32_t* __restrict__ in_root, uint64_t* __restrict__ out_part,
-                                   uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
-                                   uint32_t* __restrict__ flags) {
-  const uint32_t lane = lane_id();
-  const bool has_giant = giant != kNoRoot;
-  const uint32_t p0 = wave * kWave;
-  if (p0 >= n_entries) return;
-  const uint32_t p1 = (p0 + kWave < n_entries) ? p0 + kWave : n_entries;
-  const uint32_t len = p1 - p0;
-  const uint32_t waves_here = (n_entries + kWave - 1) / kWave;
+// Slot of a range's boundary partial: 128 u64 (alpha of lanes 0..63, then gamma) and a root.
+struct Slot {
+  uint64_t* vals;
+  uint32_t* root;
+};
 
-  auto entry_root = [&](uint32_t p) -> uint32_t { return FIRST ? vroot[members[p]] : in_root[p]; };
+// Reduce entries [p0, p0 + len) (len <= 64) with one wave.  prev_root / next_root: roots of the
+// entries just before and after the range (kNoRoot when none), which decide whether the first and
+// last segments are open.  FIRST: entries are members (rows of the slab); otherwise partial
+// entries (in_part / in_root, global or LDS).  *all_a / *all_g get the XOR of every entry read.
+template <class G, bool FIRST>
+__device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_root, uint32_t next_root,
+                                    const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
+                                    const unsigned char* __restrict__ slab, const Geo& g, uint32_t round,
+                                    const uint64_t* in_part, const uint32_t* in_root, Slot head, Slot tail,
+                                    uint64_t* __restrict__ samp, uint32_t* __restrict__ flags, uint64_t* all_a_out,
+                                    uint64_t* all_g_out) {
+  const uint32_t lane = lane_id();
   uint32_t m = 0, rt = kNoRoot;
   if (lane < len) {
     if (FIRST) {
@@ -1046,44 +1053,45 @@ __device__ inline void reduce_wave(uint32_t wave, uint32_t n_entries, uint32_t g
     }
   }
   const uint32_t first_root = rd_lane_u32(rt, 0), last_root = rd_lane_u32(rt, len - 1);
-  const bool head_open = p0 > 0 && first_root != kNoRoot && entry_root(p0 - 1) == first_root;
-  const bool tail_open = p1 < n_entries && last_root != kNoRoot && entry_root(p1) == last_root;
+  const bool head_open = first_root != kNoRoot && prev_root == first_root;
+  const bool tail_open = last_root != kNoRoot && next_root == last_root;
+  const uint32_t p1 = p0 + len;
 
   const bool in_col = lane < g.D;
   const uint32_t boff = in_col ? round * g.D + lane : (g.nb - 1);
   const bool loads = lane <= g.D;
-  uint64_t* out_a = out_part + (size_t)(2 * wave) * 2 * kWave;  // entry layout: alpha[64] then gamma[64]
   bool wrote_head = false, wrote_tail = false;
 
-  auto sample_to = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g) {
-    const uint64_t det_a = rd_lane_u64(acc_a, g.D);
-    const G det_g = (G)rd_lane_u64(acc_g, g.D);
-    uint64_t idx;
-    const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
-    if (lane == 0) samp[R] = sample_record(code, idx, g.n, flags);
+  auto put = [&](Slot sl, uint32_t R, uint64_t a, uint64_t gm) {
+    sl.vals[lane] = a;
+    sl.vals[kWave + lane] = gm;
+    if (lane == 0) *sl.root = R;
   };
   auto emit = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g, uint32_t seg_start, uint32_t seg_end) {
     const bool started_before = seg_start == p0 && head_open;
     const bool continues = seg_end == p1 && tail_open;
     if (!started_before && !continues) {
-      if (R != kNoRoot) sample_to(R, acc_a, acc_g);
+      if (R != kNoRoot) {
+        const uint64_t det_a = rd_lane_u64(acc_a, g.D);
+        const G det_g = (G)rd_lane_u64(acc_g, g.D);
+        uint64_t idx;
+        const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
+        if (lane == 0) samp[R] = sample_record(code, idx, g.n, flags);
+      }
       return;
     }
-    // Partial of a spanning supernode: the head slot if it started before this wave, else the tail
-    // slot.  When it covers the whole wave the tail slot carries the same root with a zero value,
-    // so the supernode's entries stay contiguous at the next level.
-    const uint32_t slot = started_before ? 0 : 1;
-    uint64_t* o = out_a + (size_t)slot * 2 * kWave;
-    o[lane] = acc_a;
-    o[kWave + lane] = acc_g;
-    if (lane == 0) out_root[2 * wave + slot] = R;
-    if (slot == 0) wrote_head = true;
-    else wrote_tail = true;
-    if (started_before && continues) {
-      uint64_t* t = out_a + 2 * kWave;
-      t[lane] = 0;
-      t[kWave + lane] = 0;
-      if (lane == 0) out_root[2 * wave + 1] = R;
+    // Partial of a spanning supernode: the head slot if it started before this range, else the
+    // tail slot.  When it covers the whole range the tail slot carries the same root with a zero
+    // value, so the supernode's entries stay contiguous at the next level.
+    if (started_before) {
+      put(head, R, acc_a, acc_g);
+      wrote_head = true;
+      if (continues) {
+        put(tail, R, 0, 0);
+        wrote_tail = true;
+      }
+    } else {
+      put(tail, R, acc_a, acc_g);
       wrote_tail = true;
     }
   };
@@ -1102,7 +1110,7 @@ __device__ inline void reduce_wave(uint32_t wave, uint32_t n_entries, uint32_t g
           G gw;
           ld_rec<G>(slab + rec_off<G>(g, rd_lane_u32(m, j + k), boff), &va[k], &gw);
           vg[k] = (uint64_t)gw;
-        } else if (rd_lane_u32(rt, j + k) != kNoRoot) {  // empty partial slots hold no value
+        } else if (rd_lane_u32(rt, j + k) != kNoRoot) {  // empty slots hold no value
           const uint64_t* e = in_part + (size_t)(p0 + j + k) * 2 * kWave;
           va[k] = e[lane];
           vg[k] = e[kWave + lane];
@@ -1122,65 +1130,129 @@ __device__ inline void reduce_wave(uint32_t wave, uint32_t n_entries, uint32_t g
         }
         acc_a ^= va[k];
         acc_g ^= vg[k];
-        if (FIRST) {
-          all_a ^= va[k];
-          all_g ^= vg[k];
-        }
+        all_a ^= va[k];
+        all_g ^= vg[k];
       }
     }
   }
   emit(cur, acc_a, acc_g, seg_start, p1);
-  if (FIRST && has_giant) {
-    if (waves_here == 1) {
-      sample_to(giant, all_a, all_g);  // single wave: its total is the giant's whole sum
-    } else {
-      uint64_t* t = out_part + (size_t)(2 * waves_here + wave) * 2 * kWave;
-      t[lane] = all_a;
-      t[kWave + lane] = all_g;
-      if (lane == 0) out_root[2 * waves_here + wave] = giant;
-    }
-  }
-  // unused output slots carry no root (and their values are never read)
-  if (lane == 0 && !wrote_head) out_root[2 * wave] = kNoRoot;
-  if (lane == 0 && !wrote_tail) out_root[2 * wave + 1] = kNoRoot;
+  if (lane == 0 && !wrote_head) *head.root = kNoRoot;
+  if (lane == 0 && !wrote_tail) *tail.root = kNoRoot;
+  *all_a_out = all_a;
+  *all_g_out = all_g;
 }
 
+// One workgroup of kRedBlock threads reduces entries [1024*blk, 1024*blk + 1024) of n_entries into
+// the global slots 2*blk (head) and 2*blk+1 (tail) of out_part / out_root, plus (FIRST with a
+// giant) its total in slot 2*nblk + blk, or the giant's sample when the level has one workgroup.
+// Uniform over the workgroup; contains __syncthreads.
+template <class G, bool FIRST>
+__device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t giant, const uint32_t* members,
+                                    const uint32_t* vroot, const unsigned char* slab, const Geo& g, uint32_t round,
+                                    const uint64_t* in_part, const uint32_t* in_root, uint64_t* out_part,
+                                    uint32_t* out_root, uint64_t* samp, uint32_t* flags, uint64_t* s_vals,
+                                    uint32_t* s_root, uint64_t* s_tot) {
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  const uint32_t nblk = (n_entries + kRedSpan - 1) / kRedSpan;
+  const uint32_t b0 = blk * kRedSpan;
+  const uint32_t b1 = b0 + kRedSpan < n_entries ? b0 + kRedSpan : n_entries;
+  auto root_at = [&](uint32_t p) -> uint32_t { return FIRST ? vroot[members[p]] : in_root[p]; };
+  const bool giant_tot = FIRST && giant != kNoRoot;
+  if (giant_tot)
+    for (uint32_t k = threadIdx.x; k < 2 * kWave; k += blockDim.x) s_tot[k] = 0;
+  __syncthreads();
+  // phase 1: every wave reduces its 64 entries, leaving its boundary partials in LDS slots
+  const uint32_t p0 = b0 + wid * kWave;
+  Slot hs{s_vals + (size_t)(2 * wid) * 2 * kWave, s_root + 2 * wid};
+  Slot ts{s_vals + (size_t)(2 * wid + 1) * 2 * kWave, s_root + 2 * wid + 1};
+  if (p0 < b1) {
+    const uint32_t len = b1 - p0 < kWave ? b1 - p0 : kWave;
+    const uint32_t prev = p0 > 0 ? root_at(p0 - 1) : kNoRoot;
+    const uint32_t next = p0 + len < n_entries ? root_at(p0 + len) : kNoRoot;
+    uint64_t wa, wg;
+    reduce_range<G, FIRST>(p0, len, prev, next, members, vroot, slab, g, round, in_part, in_root, hs, ts, samp, flags,
+                           &wa, &wg);
+    if (giant_tot) {
+      lds_xor(s_tot + lane, wa);
+      lds_xor(s_tot + kWave + lane, wg);
+    }
+  } else if (lane == 0) {
+    *hs.root = kNoRoot;
+    *ts.root = kNoRoot;
+  }
+  __syncthreads();
+  // phase 2: wave 0 reduces the workgroup's 32 slots into its two global slots
+  if (wid == 0) {
+    const uint32_t prev = b0 > 0 ? root_at(b0 - 1) : kNoRoot;
+    const uint32_t next = b1 < n_entries ? root_at(b1) : kNoRoot;
+    Slot gh{out_part + (size_t)(2 * blk) * 2 * kWave, out_root + 2 * blk};
+    Slot gt{out_part + (size_t)(2 * blk + 1) * 2 * kWave, out_root + 2 * blk + 1};
+    uint64_t wa, wg;
+    reduce_range<G, false>(0, 2 * kRedWaves, prev, next, nullptr, nullptr, nullptr, g, round, s_vals, s_root, gh, gt,
+                           samp, flags, &wa, &wg);
+    if (giant_tot) {
+      const uint64_t ta = s_tot[lane], tg = s_tot[kWave + lane];
+      if (nblk == 1) {  // one workgroup: its total is the giant's whole sum
+        const bool in_col = lane < g.D;
+        const uint64_t det_a = rd_lane_u64(ta, g.D);
+        const G det_g = (G)rd_lane_u64(tg, g.D);
+        uint64_t idx;
+        const int code = wave_sample<G>(det_a, det_g, ta, (G)tg, in_col, g.chk_bf, &idx);
+        if (lane == 0) samp[giant] = sample_record(code, idx, g.n, flags);
+      } else {
+        Slot tt{out_part + (size_t)(2 * nblk + blk) * 2 * kWave, out_root + 2 * nblk + blk};
+        tt.vals[lane] = ta;
+        tt.vals[kWave + lane] = tg;
+        if (lane == 0) *tt.root = giant;
+      }
+    }
+  }
+  __syncthreads();  // the LDS slots are reused by the caller's next call
+}
+
+// LDS of one reduce workgroup: 32 slots of 128 u64, their roots, the giant total.
+#define GZ_REDUCE_LDS                                             \
+  __shared__ uint64_t s_vals[2 * kRedWaves * 2 * kWave];          \
+  __shared__ uint32_t s_root[2 * kRedWaves];                      \
+  __shared__ uint64_t s_tot[2 * kWave];
 
 template <class G>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRedBlock)
 k_bv_reduce_first(const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
                   const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
                   uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
                   uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats) {
   if (!ld_acq(prev_mod)) return;
+  GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
   const uint32_t e1 = st[kStSeg + (round & 1)];
   if (rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
     rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
     rstats[2 * round + 1] = giant;
   }
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   if (giant != kNoRoot && e1 == 0) {  // every vertex is in the giant: its sum is zero
-    if (wave == 0 && lane_id() == 0) samp[giant] = kSampZero;
+    if (blockIdx.x == 0 && threadIdx.x == 0) samp[giant] = kSampZero;
     return;
   }
-  reduce_wave<G, true>(wave, e1, giant, members, vroot, slab, g, round, nullptr, nullptr, out_part, out_root, samp,
-                       flags);
+  if (blockIdx.x * kRedSpan >= e1) return;  // uniform over the workgroup
+  reduce_block<G, true>(blockIdx.x, e1, giant, members, vroot, slab, g, round, nullptr, nullptr, out_part, out_root,
+                        samp, flags, s_vals, s_root, s_tot);
 }
 
-// Levels >= 2 of the reduce.  Every workgroup takes its waves of level `level`.  With `tail`
-// (the host sets it once the following levels hold at most kTailEntries entries), the last
+// Levels >= 2 of the reduce.  Every workgroup takes its 1024 entries of level `level`.  With
+// `tail` (the host sets it once the following levels hold at most kTailEntries entries), the last
 // workgroup to finish -- a counter per round with device-scope fences around it, so no workgroup
-// ever waits -- then runs the remaining levels alone, so a round needs one kernel for its small
-// upper levels.  Level l reads part[l & 1] and writes part[(l - 1) & 1].
-constexpr uint32_t kTailEntries = 1024;
+// ever waits -- then runs the remaining levels alone.  Level l reads part[l & 1] and writes
+// part[(l - 1) & 1].
+constexpr uint32_t kTailEntries = 4 * kRedSpan;
 template <class G>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRedBlock)
 k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const Geo g, uint32_t round,
                uint64_t* __restrict__ part0, uint64_t* __restrict__ part1, uint32_t* __restrict__ proot0,
                uint32_t* __restrict__ proot1, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
                const uint32_t* __restrict__ prev_mod) {
   if (!ld_acq(prev_mod)) return;
+  GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
   const bool has_giant = giant != kNoRoot;
   const uint32_t e1 = st[kStSeg + (round & 1)];
@@ -1189,9 +1261,10 @@ k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const 
   if (!level_entries(e1, has_giant, level0, &ne)) return;  // uniform over the grid: an earlier level was the last
   uint64_t* part[2] = {part0, part1};
   uint32_t* proot[2] = {proot0, proot1};
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  reduce_wave<G, false>(wave, ne, giant, nullptr, nullptr, nullptr, g, round, part[level0 & 1], proot[level0 & 1],
-                        part[(level0 - 1) & 1], proot[(level0 - 1) & 1], samp, flags);
+  if (blockIdx.x * kRedSpan < ne)
+    reduce_block<G, false>(blockIdx.x, ne, giant, nullptr, nullptr, nullptr, g, round, part[level0 & 1],
+                           proot[level0 & 1], part[(level0 - 1) & 1], proot[(level0 - 1) & 1], samp, flags, s_vals,
+                           s_root, s_tot);
   if (!tail) return;
   __shared__ uint32_t s_last;
   __threadfence();
@@ -1200,13 +1273,12 @@ k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const 
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  const uint32_t wid = threadIdx.x / kWave, nwb = blockDim.x / kWave;
   for (uint32_t level = level0 + 1;; ++level) {
     if (!level_entries(e1, has_giant, level, &ne)) break;
-    const uint32_t nw = (ne + kWave - 1) / kWave;
-    for (uint32_t w = wid; w < nw; w += nwb)
-      reduce_wave<G, false>(w, ne, giant, nullptr, nullptr, nullptr, g, round, part[level & 1], proot[level & 1],
-                            part[(level - 1) & 1], proot[(level - 1) & 1], samp, flags);
+    const uint32_t nb = (ne + kRedSpan - 1) / kRedSpan;
+    for (uint32_t blk = 0; blk < nb; ++blk)
+      reduce_block<G, false>(blk, ne, giant, nullptr, nullptr, nullptr, g, round, part[level & 1], proot[level & 1],
+                             part[(level - 1) & 1], proot[(level - 1) & 1], samp, flags, s_vals, s_root, s_tot);
     __threadfence_block();
     __syncthreads();
   }
@@ -1473,8 +1545,8 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
 }
 
 // Host bound on each level's entry count (the kernels compute the exact counts on the device):
-// the larger of the chain without a supernode summed from its complement (level 2 = 2 * W1) and
-// with one (level 2 = 3 * W1); e1 <= n in both.
+// the larger of the chain without a supernode summed from its complement (level 2 = 2 * B1) and
+// with one (level 2 = 3 * B1); e1 <= n in both.
 size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) {
   uint32_t a = n, b = n;
   size_t L = 0;
@@ -1482,7 +1554,7 @@ size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) 
   while (true) {
     if (L < max_levels && entries_per_level) entries_per_level[L] = std::max(a_done ? 0u : a, b_done ? 0u : b);
     ++L;
-    const uint32_t wa = (a + kWave - 1) / kWave, wb = (b + kWave - 1) / kWave;
+    const uint32_t wa = (a + kRedSpan - 1) / kRedSpan, wb = (b + kRedSpan - 1) / kRedSpan;
     if (wa <= 1) a_done = true;
     if (wb <= 1) b_done = true;
     if (a_done && b_done) break;
@@ -1509,23 +1581,23 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
                      act_list, prev);
   uint32_t ent[32];
   const size_t L = round_levels(n, ent, 32);
-  const uint32_t blocks1 = std::max(1u, (ent[0] + 255) / 256);  // one wave per 64 entries
+  const uint32_t blocks1 = std::max(1u, (ent[0] + kRedSpan - 1) / kRedSpan);  // 1024 entries per workgroup
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_bv_reduce_first<uint32_t>, dim3(blocks1), dim3(256), 0, s, st, members, rb.root, slab, g,
-                       round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
+    hipLaunchKernelGGL(k_bv_reduce_first<uint32_t>, dim3(blocks1), dim3(kRedBlock), 0, s, st, members, rb.root, slab,
+                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
   else
-    hipLaunchKernelGGL(k_bv_reduce_first<uint64_t>, dim3(blocks1), dim3(256), 0, s, st, members, rb.root, slab, g,
-                       round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
+    hipLaunchKernelGGL(k_bv_reduce_first<uint64_t>, dim3(blocks1), dim3(kRedBlock), 0, s, st, members, rb.root, slab,
+                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
   for (size_t l = 1; l < L; ++l) {  // level l + 1
-    const uint32_t blocks = std::max(1u, (ent[l] + 255) / 256);
+    const uint32_t blocks = std::max(1u, (ent[l] + kRedSpan - 1) / kRedSpan);
     const bool tail = l + 1 >= L || ent[l + 1] <= kTailEntries;
     const uint32_t level = (uint32_t)l + 1;
     if (gbytes == 4)
-      hipLaunchKernelGGL(k_bv_reduce_up<uint32_t>, dim3(blocks), dim3(256), 0, s, level, tail ? 1u : 0u, st, g, round,
-                         rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
+      hipLaunchKernelGGL(k_bv_reduce_up<uint32_t>, dim3(blocks), dim3(kRedBlock), 0, s, level, tail ? 1u : 0u, st, g,
+                         round, rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
     else
-      hipLaunchKernelGGL(k_bv_reduce_up<uint64_t>, dim3(blocks), dim3(256), 0, s, level, tail ? 1u : 0u, st, g, round,
-                         rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
+      hipLaunchKernelGGL(k_bv_reduce_up<uint64_t>, dim3(blocks), dim3(kRedBlock), 0, s, level, tail ? 1u : 0u, st, g,
+                         round, rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
     if (tail) break;
   }
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
