@@ -736,7 +736,7 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
 template <class G>
 __global__ void __launch_bounds__(256)
 k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restrict__ samp,
-                uint32_t* __restrict__ flags) {
+                uint32_t* __restrict__ flags, unsigned char* __restrict__ det_cpy) {
   constexpr uint32_t V = 4;  // vertices per wave iteration: all their loads are issued first
   const uint32_t lane = lane_id();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
@@ -761,6 +761,8 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
       uint64_t idx;
       const int code = wave_sample<G>(det_a[k], det_g[k], a[k], gm[k], valid, g.chk_bf, &idx);
       if (lane == 0) samp[v0 + k] = sample_record(code, idx, g.n, flags);
+      // the deterministic bucket does not depend on the round: later rounds read this compact copy
+      if (lane == 1) st_rec<G>(det_cpy + (size_t)(v0 + k) * (8 + sizeof(G)), det_a[k], det_g[k]);
     }
   }
 }
@@ -1012,12 +1014,19 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
 // Entry counts are on the device: level 1 has e1 = (entries regrouped) entries; level 2 has
 // 2*B1 (+ B1 totals with a giant) with B1 = ceil(e1/1024); later levels 2*ceil(e/1024).  A level
 // whose predecessor had a single workgroup is a no-op.
-constexpr uint32_t kRedWaves = 16, kRedBlock = kRedWaves * kWave, kRedSpan = kRedWaves * kWave;
+constexpr uint32_t kRedWaves = 16, kRedBlock = kRedWaves * kWave;
+// entries per wave: 64 rows at level 1; 16 partials above, one batch of loads, so the small upper
+// levels spread over more waves instead of chaining dependent batches in a few
+constexpr uint32_t kUpPerWave = 16;
+__host__ __device__ inline uint32_t level_span(uint32_t level) {
+  return kRedWaves * (level == 1 ? kWave : kUpPerWave);
+}
 
 __device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, uint32_t* e) {
   uint32_t cur = e1;
   for (uint32_t l = 1; l < level; ++l) {
-    const uint32_t b = (cur + kRedSpan - 1) / kRedSpan;
+    const uint32_t span = level_span(l);
+    const uint32_t b = (cur + span - 1) / span;
     if (b <= 1) return false;
     cur = 2 * b + ((l == 1 && giant) ? b : 0);
   }
@@ -1041,7 +1050,7 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
                                     const unsigned char* __restrict__ slab, const Geo& g, uint32_t round,
                                     const uint64_t* in_part, const uint32_t* in_root, Slot head, Slot tail,
                                     uint64_t* __restrict__ samp, uint32_t* __restrict__ flags, uint64_t* all_a_out,
-                                    uint64_t* all_g_out) {
+                                    uint64_t* all_g_out, const unsigned char* __restrict__ det_cpy) {
   const uint32_t lane = lane_id();
   uint32_t m = 0, rt = kNoRoot;
   if (lane < len) {
@@ -1107,8 +1116,13 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
       vg[k] = 0;
       if (j + k < len && loads) {
         if (FIRST) {
+          // lanes < D read the column's records, lane D the compact deterministic bucket: one
+          // load instruction with per-lane addresses
+          const uint32_t mv = rd_lane_u32(m, j + k);
+          const unsigned char* src = in_col ? slab + rec_off<G>(g, mv, boff)
+                                            : reinterpret_cast<const unsigned char*>(det_cpy) + (size_t)mv * (8 + sizeof(G));
           G gw;
-          ld_rec<G>(slab + rec_off<G>(g, rd_lane_u32(m, j + k), boff), &va[k], &gw);
+          ld_rec<G>(src, &va[k], &gw);
           vg[k] = (uint64_t)gw;
         } else if (rd_lane_u32(rt, j + k) != kNoRoot) {  // empty slots hold no value
           const uint64_t* e = in_part + (size_t)(p0 + j + k) * 2 * kWave;
@@ -1151,27 +1165,29 @@ __device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t g
                                     const uint32_t* vroot, const unsigned char* slab, const Geo& g, uint32_t round,
                                     const uint64_t* in_part, const uint32_t* in_root, uint64_t* out_part,
                                     uint32_t* out_root, uint64_t* samp, uint32_t* flags, uint64_t* s_vals,
-                                    uint32_t* s_root, uint64_t* s_tot) {
+                                    uint32_t* s_root, uint64_t* s_tot, const unsigned char* det_cpy) {
+  constexpr uint32_t per_wave = FIRST ? kWave : kUpPerWave;
+  constexpr uint32_t span = kRedWaves * per_wave;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
-  const uint32_t nblk = (n_entries + kRedSpan - 1) / kRedSpan;
-  const uint32_t b0 = blk * kRedSpan;
-  const uint32_t b1 = b0 + kRedSpan < n_entries ? b0 + kRedSpan : n_entries;
+  const uint32_t nblk = (n_entries + span - 1) / span;
+  const uint32_t b0 = blk * span;
+  const uint32_t b1 = b0 + span < n_entries ? b0 + span : n_entries;
   auto root_at = [&](uint32_t p) -> uint32_t { return FIRST ? vroot[members[p]] : in_root[p]; };
   const bool giant_tot = FIRST && giant != kNoRoot;
   if (giant_tot)
     for (uint32_t k = threadIdx.x; k < 2 * kWave; k += blockDim.x) s_tot[k] = 0;
   __syncthreads();
-  // phase 1: every wave reduces its 64 entries, leaving its boundary partials in LDS slots
-  const uint32_t p0 = b0 + wid * kWave;
+  // phase 1: every wave reduces its entries, leaving its boundary partials in LDS slots
+  const uint32_t p0 = b0 + wid * per_wave;
   Slot hs{s_vals + (size_t)(2 * wid) * 2 * kWave, s_root + 2 * wid};
   Slot ts{s_vals + (size_t)(2 * wid + 1) * 2 * kWave, s_root + 2 * wid + 1};
   if (p0 < b1) {
-    const uint32_t len = b1 - p0 < kWave ? b1 - p0 : kWave;
+    const uint32_t len = b1 - p0 < per_wave ? b1 - p0 : per_wave;
     const uint32_t prev = p0 > 0 ? root_at(p0 - 1) : kNoRoot;
     const uint32_t next = p0 + len < n_entries ? root_at(p0 + len) : kNoRoot;
     uint64_t wa, wg;
     reduce_range<G, FIRST>(p0, len, prev, next, members, vroot, slab, g, round, in_part, in_root, hs, ts, samp, flags,
-                           &wa, &wg);
+                           &wa, &wg, det_cpy);
     if (giant_tot) {
       lds_xor(s_tot + lane, wa);
       lds_xor(s_tot + kWave + lane, wg);
@@ -1189,7 +1205,7 @@ __device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t g
     Slot gt{out_part + (size_t)(2 * blk + 1) * 2 * kWave, out_root + 2 * blk + 1};
     uint64_t wa, wg;
     reduce_range<G, false>(0, 2 * kRedWaves, prev, next, nullptr, nullptr, nullptr, g, round, s_vals, s_root, gh, gt,
-                           samp, flags, &wa, &wg);
+                           samp, flags, &wa, &wg, nullptr);
     if (giant_tot) {
       const uint64_t ta = s_tot[lane], tg = s_tot[kWave + lane];
       if (nblk == 1) {  // one workgroup: its total is the giant's whole sum
@@ -1221,7 +1237,8 @@ __global__ void __launch_bounds__(kRedBlock)
 k_bv_reduce_first(const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
                   const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
                   uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
-                  uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats) {
+                  uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats,
+                  const unsigned char* __restrict__ det_cpy) {
   if (!ld_acq(prev_mod)) return;
   GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
@@ -1234,9 +1251,9 @@ k_bv_reduce_first(const uint32_t* __restrict__ st, const uint32_t* __restrict__ 
     if (blockIdx.x == 0 && threadIdx.x == 0) samp[giant] = kSampZero;
     return;
   }
-  if (blockIdx.x * kRedSpan >= e1) return;  // uniform over the workgroup
+  if (blockIdx.x * level_span(1) >= e1) return;  // uniform over the workgroup
   reduce_block<G, true>(blockIdx.x, e1, giant, members, vroot, slab, g, round, nullptr, nullptr, out_part, out_root,
-                        samp, flags, s_vals, s_root, s_tot);
+                        samp, flags, s_vals, s_root, s_tot, det_cpy);
 }
 
 // Levels >= 2 of the reduce.  Every workgroup takes its 1024 entries of level `level`.  With
@@ -1244,7 +1261,7 @@ k_bv_reduce_first(const uint32_t* __restrict__ st, const uint32_t* __restrict__ 
 // workgroup to finish -- a counter per round with device-scope fences around it, so no workgroup
 // ever waits -- then runs the remaining levels alone.  Level l reads part[l & 1] and writes
 // part[(l - 1) & 1].
-constexpr uint32_t kTailEntries = 4 * kRedSpan;
+constexpr uint32_t kTailEntries = 4 * kRedWaves * kUpPerWave;
 template <class G>
 __global__ void __launch_bounds__(kRedBlock)
 k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const Geo g, uint32_t round,
@@ -1261,10 +1278,10 @@ k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const 
   if (!level_entries(e1, has_giant, level0, &ne)) return;  // uniform over the grid: an earlier level was the last
   uint64_t* part[2] = {part0, part1};
   uint32_t* proot[2] = {proot0, proot1};
-  if (blockIdx.x * kRedSpan < ne)
+  if (blockIdx.x * level_span(level0) < ne)
     reduce_block<G, false>(blockIdx.x, ne, giant, nullptr, nullptr, nullptr, g, round, part[level0 & 1],
                            proot[level0 & 1], part[(level0 - 1) & 1], proot[(level0 - 1) & 1], samp, flags, s_vals,
-                           s_root, s_tot);
+                           s_root, s_tot, nullptr);
   if (!tail) return;
   __shared__ uint32_t s_last;
   __threadfence();
@@ -1275,10 +1292,11 @@ k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const 
   __threadfence();
   for (uint32_t level = level0 + 1;; ++level) {
     if (!level_entries(e1, has_giant, level, &ne)) break;
-    const uint32_t nb = (ne + kRedSpan - 1) / kRedSpan;
+    const uint32_t nb = (ne + level_span(level) - 1) / level_span(level);
     for (uint32_t blk = 0; blk < nb; ++blk)
       reduce_block<G, false>(blk, ne, giant, nullptr, nullptr, nullptr, g, round, part[level & 1], proot[level & 1],
-                             part[(level - 1) & 1], proot[(level - 1) & 1], samp, flags, s_vals, s_root, s_tot);
+                             part[(level - 1) & 1], proot[(level - 1) & 1], samp, flags, s_vals, s_root, s_tot,
+                             nullptr);
     __threadfence_block();
     __syncthreads();
   }
@@ -1537,9 +1555,9 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
   uint64_t* samp = rb.samp;
   const uint32_t grid = grid_for((size_t)(g.n + 3) / 4 * kWave, 256, 16384);
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags);
+    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy);
   else
-    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags);
+    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr, 0u,
                      nullptr, nullptr, d.mod);
 }
@@ -1554,7 +1572,8 @@ size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) 
   while (true) {
     if (L < max_levels && entries_per_level) entries_per_level[L] = std::max(a_done ? 0u : a, b_done ? 0u : b);
     ++L;
-    const uint32_t wa = (a + kRedSpan - 1) / kRedSpan, wb = (b + kRedSpan - 1) / kRedSpan;
+    const uint32_t span = level_span((uint32_t)L);
+    const uint32_t wa = (a + span - 1) / span, wb = (b + span - 1) / span;
     if (wa <= 1) a_done = true;
     if (wb <= 1) b_done = true;
     if (a_done && b_done) break;
@@ -1581,15 +1600,17 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
                      act_list, prev);
   uint32_t ent[32];
   const size_t L = round_levels(n, ent, 32);
-  const uint32_t blocks1 = std::max(1u, (ent[0] + kRedSpan - 1) / kRedSpan);  // 1024 entries per workgroup
+  const uint32_t blocks1 = std::max(1u, (ent[0] + level_span(1) - 1) / level_span(1));
   if (gbytes == 4)
     hipLaunchKernelGGL(k_bv_reduce_first<uint32_t>, dim3(blocks1), dim3(kRedBlock), 0, s, st, members, rb.root, slab,
-                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
+                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats,
+                       rb.det_cpy);
   else
     hipLaunchKernelGGL(k_bv_reduce_first<uint64_t>, dim3(blocks1), dim3(kRedBlock), 0, s, st, members, rb.root, slab,
-                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats);
+                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats,
+                       rb.det_cpy);
   for (size_t l = 1; l < L; ++l) {  // level l + 1
-    const uint32_t blocks = std::max(1u, (ent[l] + kRedSpan - 1) / kRedSpan);
+    const uint32_t blocks = std::max(1u, (ent[l] + level_span((uint32_t)l + 1) - 1) / level_span((uint32_t)l + 1));
     const bool tail = l + 1 >= L || ent[l + 1] <= kTailEntries;
     const uint32_t level = (uint32_t)l + 1;
     if (gbytes == 4)

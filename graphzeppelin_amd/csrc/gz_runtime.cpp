@@ -173,7 +173,7 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
   // forest edge (8 each), partial entries (2 buffers x 3 x 1 KiB per 64 vertices), spanning-forest
   // hash set (16)
-  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 96 + 16) + 8192;
+  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 96 + 16 + 8 + gbytes) + 8192;  // + deterministic copy
   return slab + ingest + rounds;
 }
 
@@ -228,6 +228,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   r_members[0].alloc(n); r_members[1].alloc(n);
   r_totals.alloc(8);
   r_stats.alloc(128);
+  r_det.alloc((size_t)n * (8 + gb));
   launch_fill_u32(r_stats.p, 0, 128, st);
   for (int i = 0; i < 2; ++i) {
     // level 1 writes head and tail partials per wave, plus one total per wave when a supernode
@@ -236,7 +237,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
     r_proot[i].alloc(3 * waves1 + 2);
   }
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
-                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p, r_stats.p};
+                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p, r_stats.p, r_det.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
