@@ -1035,6 +1035,7 @@ __device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, ui
 }
 
 // Slot of a range's boundary partial: 128 u64 (alpha of lanes 0..63, then gamma) and a root.
+constexpr uint32_t kNeighborLoad = 0xFFFFFFFEu;  // reduce_range: load the neighbour root itself
 struct Slot {
   uint64_t* vals;
   uint32_t* root;
@@ -1045,7 +1046,7 @@ struct Slot {
 // last segments are open.  FIRST: entries are members (rows of the slab); otherwise partial
 // entries (in_part / in_root, global or LDS).  *all_a / *all_g get the XOR of every entry read.
 template <class G, bool FIRST>
-__device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_root, uint32_t next_root,
+__device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_root_in, uint32_t next_root_in,
                                     const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
                                     const unsigned char* __restrict__ slab, const Geo& g, uint32_t round,
                                     const uint64_t* in_part, const uint32_t* in_root, Slot head, Slot tail,
@@ -1053,6 +1054,13 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
                                     uint64_t* all_g_out, const unsigned char* __restrict__ det_cpy) {
   const uint32_t lane = lane_id();
   uint32_t m = 0, rt = kNoRoot;
+  // prev_root_in / next_root_in == kNeighborLoad: the neighbours are entries p0 - 1 and p0 + len of
+  // in_root, loaded together with the range's roots (no extra round trip)
+  uint32_t prev_root = prev_root_in, next_root = next_root_in;
+  if (!FIRST) {
+    if (prev_root_in == kNeighborLoad) prev_root = in_root[p0 - 1];
+    if (next_root_in == kNeighborLoad) next_root = in_root[p0 + len];
+  }
   if (lane < len) {
     if (FIRST) {
       m = members[p0 + lane];
@@ -1124,7 +1132,7 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
           G gw;
           ld_rec<G>(src, &va[k], &gw);
           vg[k] = (uint64_t)gw;
-        } else if (rd_lane_u32(rt, j + k) != kNoRoot) {  // empty slots hold no value
+        } else {  // empty slots hold zeros, so the values need not wait for the roots
           const uint64_t* e = in_part + (size_t)(p0 + j + k) * 2 * kWave;
           va[k] = e[lane];
           vg[k] = e[kWave + lane];
@@ -1150,8 +1158,9 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
     }
   }
   emit(cur, acc_a, acc_g, seg_start, p1);
-  if (lane == 0 && !wrote_head) *head.root = kNoRoot;
-  if (lane == 0 && !wrote_tail) *tail.root = kNoRoot;
+  // an unused slot: no root, zero value (the next level reads values without looking at roots)
+  if (!wrote_head) put(head, kNoRoot, 0, 0);
+  if (!wrote_tail) put(tail, kNoRoot, 0, 0);
   *all_a_out = all_a;
   *all_g_out = all_g;
 }
@@ -1174,6 +1183,9 @@ __device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t g
   const uint32_t b1 = b0 + span < n_entries ? b0 + span : n_entries;
   auto root_at = [&](uint32_t p) -> uint32_t { return FIRST ? vroot[members[p]] : in_root[p]; };
   const bool giant_tot = FIRST && giant != kNoRoot;
+  // neighbours of the workgroup's range, for phase 2; loaded early so they are in flight with phase 1
+  const uint32_t blk_prev = (wid == 0 && b0 > 0) ? root_at(b0 - 1) : kNoRoot;
+  const uint32_t blk_next = (wid == 0 && b1 < n_entries) ? root_at(b1) : kNoRoot;
   if (giant_tot)
     for (uint32_t k = threadIdx.x; k < 2 * kWave; k += blockDim.x) s_tot[k] = 0;
   __syncthreads();
@@ -1183,8 +1195,8 @@ __device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t g
   Slot ts{s_vals + (size_t)(2 * wid + 1) * 2 * kWave, s_root + 2 * wid + 1};
   if (p0 < b1) {
     const uint32_t len = b1 - p0 < per_wave ? b1 - p0 : per_wave;
-    const uint32_t prev = p0 > 0 ? root_at(p0 - 1) : kNoRoot;
-    const uint32_t next = p0 + len < n_entries ? root_at(p0 + len) : kNoRoot;
+    const uint32_t prev = p0 == 0 ? kNoRoot : (FIRST ? root_at(p0 - 1) : kNeighborLoad);
+    const uint32_t next = p0 + len >= n_entries ? kNoRoot : (FIRST ? root_at(p0 + len) : kNeighborLoad);
     uint64_t wa, wg;
     reduce_range<G, FIRST>(p0, len, prev, next, members, vroot, slab, g, round, in_part, in_root, hs, ts, samp, flags,
                            &wa, &wg, det_cpy);
@@ -1192,15 +1204,20 @@ __device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t g
       lds_xor(s_tot + lane, wa);
       lds_xor(s_tot + kWave + lane, wg);
     }
-  } else if (lane == 0) {
-    *hs.root = kNoRoot;
-    *ts.root = kNoRoot;
+  } else {
+    hs.vals[lane] = 0;
+    hs.vals[kWave + lane] = 0;
+    ts.vals[lane] = 0;
+    ts.vals[kWave + lane] = 0;
+    if (lane == 0) {
+      *hs.root = kNoRoot;
+      *ts.root = kNoRoot;
+    }
   }
   __syncthreads();
   // phase 2: wave 0 reduces the workgroup's 32 slots into its two global slots
   if (wid == 0) {
-    const uint32_t prev = b0 > 0 ? root_at(b0 - 1) : kNoRoot;
-    const uint32_t next = b1 < n_entries ? root_at(b1) : kNoRoot;
+    const uint32_t prev = blk_prev, next = blk_next;
     Slot gh{out_part + (size_t)(2 * blk) * 2 * kWave, out_root + 2 * blk};
     Slot gt{out_part + (size_t)(2 * blk + 1) * 2 * kWave, out_root + 2 * blk + 1};
     uint64_t wa, wg;
