@@ -816,10 +816,11 @@ __global__ void __launch_bounds__(256)
 k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ root, DsuDev d,
                 const uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
                 const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod) {
-  if (prev_mod && !ld_acq(prev_mod)) return;
+  const uint32_t live = prev_mod ? ld_acq(prev_mod) : 1u;  // checked after the first reads
   const bool act = st && st[kStActMode] != 0;
   const uint32_t gr = act ? st[kStGiantRound] : kNoRoot;
   const uint32_t count = st ? walk_count(st, round, d.n) : d.n;
+  if (!live) return;
   bool modified = false;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const uint32_t v = walk_vertex(act_list, round, i);
@@ -857,17 +858,21 @@ __global__ void __launch_bounds__(kAggBlock)
 k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root, uint32_t* __restrict__ cnt,
                 uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
                 const uint32_t* __restrict__ prev_mod) {
-  if (!ld_acq(prev_mod)) return;
-  __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable];
-  for (uint32_t i = threadIdx.x; i < kAggTable; i += kAggBlock) {
-    t_key[i] = kEmptyKey;
-    t_cnt[i] = 0;
-  }
+  // The round flag is loaded together with the first reads (all in bounds whether or not the
+  // round runs: counts and lists are those of an earlier round of this query), saving a round trip.
+  const uint32_t live = ld_acq(prev_mod);
   uint32_t gfirst = st[kStGiantFirst];
-  if (gfirst == kNoRoot) {  // a candidate recorded by the previous round (0 when none)
-    const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
-    if (best) gfirst = (uint32_t)best;
+  const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
+  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
+  if (!live) return;
+  __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable];
+  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock) {
+    t_key[k] = kEmptyKey;
+    t_cnt[k] = 0;
   }
+  if (gfirst == kNoRoot && best) gfirst = (uint32_t)best;  // a candidate recorded by the previous round
   const bool act = gfirst != kNoRoot;
   const uint32_t gcur = act ? dsu_find_stable(parent, gfirst) : kNoRoot;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -879,9 +884,6 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
   }
   __syncthreads();
   const uint32_t lane = lane_id();
-  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n);
-  const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
   uint32_t r = 0;
   if (in_walk) {
     r = dsu_find_stable(parent, v);
@@ -911,7 +913,7 @@ __global__ void __launch_bounds__(kAggBlock)
 k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
            uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list, uint32_t giant_ok,
            const uint32_t* __restrict__ prev_mod) {
-  if (!ld_acq(prev_mod)) return;
+  const uint32_t live = ld_acq(prev_mod);  // checked after the first reads (see k_bv_find_count)
   __shared__ uint64_t wsum[kAggBlock / kWave];
   __shared__ uint32_t s_base;
   const bool act = st[kStActMode] != 0;
@@ -920,6 +922,7 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
   const bool in_walk = i < walk_count(st, round, n);
   const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
   uint32_t c = in_walk ? cnt[v] : 0;
+  if (!live) return;
   if (c) cnt[v] = 0;
   if (!act && giant_ok && 2ull * c > n) {
     st[kStGiantFirst] = v;
@@ -949,20 +952,21 @@ __global__ void __launch_bounds__(kAggBlock)
 k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
              uint32_t* __restrict__ members, const uint32_t* __restrict__ st, uint32_t round,
              const uint32_t* __restrict__ act_list, const uint32_t* __restrict__ prev_mod) {
-  if (!ld_acq(prev_mod)) return;
+  const uint32_t live = ld_acq(prev_mod);  // checked after the first reads (see k_bv_find_count)
+  const uint32_t gr = st[kStGiantRound];
+  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
+  const uint32_t r = in_walk ? root[v] : 0;
+  if (!live) return;
   __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable], t_base[kAggTable];
   for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock) {
     t_key[k] = kEmptyKey;
     t_cnt[k] = 0;
   }
   __syncthreads();
-  const uint32_t gr = st[kStGiantRound];
   const uint32_t lane = lane_id();
   const uint64_t lt = (1ull << lane) - 1;
-  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n);
-  const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
-  const uint32_t r = in_walk ? root[v] : 0;
   const bool active = in_walk && r != gr;  // the giant supernode's members are not regrouped
   uint32_t my_slot = 0, my_pos = 0;
   uint64_t todo = __ballot(active);
@@ -1256,10 +1260,11 @@ k_bv_reduce_first(const uint32_t* __restrict__ st, const uint32_t* __restrict__ 
                   uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
                   uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats,
                   const unsigned char* __restrict__ det_cpy) {
-  if (!ld_acq(prev_mod)) return;
+  const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
   GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
   const uint32_t e1 = st[kStSeg + (round & 1)];
+  if (!live) return;
   if (rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
     rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
     rstats[2 * round + 1] = giant;
@@ -1285,11 +1290,12 @@ k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const 
                uint64_t* __restrict__ part0, uint64_t* __restrict__ part1, uint32_t* __restrict__ proot0,
                uint32_t* __restrict__ proot1, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
                const uint32_t* __restrict__ prev_mod) {
-  if (!ld_acq(prev_mod)) return;
+  const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
   GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
   const bool has_giant = giant != kNoRoot;
   const uint32_t e1 = st[kStSeg + (round & 1)];
+  if (!live) return;
   if (has_giant && e1 == 0) return;
   uint32_t ne;
   if (!level_entries(e1, has_giant, level0, &ne)) return;  // uniform over the grid: an earlier level was the last
