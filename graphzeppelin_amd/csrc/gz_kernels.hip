@@ -74,9 +74,14 @@ __device__ inline void st_rlx(T* p, T v) {
 // updates touch, are then adjacent in one cache line with their checksums.  A 12-byte record is
 // only 4-byte aligned, so it is moved as 32-bit words, and a 64-bit XOR atomic on it becomes two
 // 32-bit ones (XOR acts bit by bit, so the result is the same).
+// Physical order within a row: the deterministic bucket (reference index nb - 1) first, then the
+// columns in reference order, so the deterministic bucket and column 0 -- all that round 0 reads,
+// and the records every update touches most -- are one contiguous run.  rec_off takes the
+// reference bucket index.
+__host__ __device__ inline uint32_t phys_slot(const Geo& g, uint32_t b) { return b == g.nb - 1 ? 0 : b + 1; }
 template <class G>
 __device__ inline size_t rec_off(const Geo& g, uint32_t v, uint32_t b) {
-  return (size_t)v * g.row + (size_t)b * (8 + sizeof(G));
+  return (size_t)v * g.row + (size_t)phys_slot(g, b) * (8 + sizeof(G));
 }
 template <class G>
 __device__ inline void ld_rec(const unsigned char* p, uint64_t* a, G* gm) {
@@ -344,7 +349,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
               lds_xor(s_a + li, idx);
               lds_xor(s_g + li, chk);
             } else {
-              xor_rec_atomic<G>(row + (size_t)(c * D + depth) * RB, idx, chk);
+              xor_rec_atomic<G>(row + (size_t)(c * D + depth + 1) * RB, idx, chk);  // physical slot
             }
           }
         }
@@ -385,8 +390,9 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
         bi[k] = 0;
         if (e < lds_n) {
           const uint32_t cc = e / kLdsPitch, d = e - cc * kLdsPitch;
-          // entries at the pitch padding (d == kLdsDepth) or at depth >= D are never written
-          bi[k] = e == det_slot ? C * D : cc * D + d;
+          // entries at the pitch padding (d == kLdsDepth) or at depth >= D are never written;
+          // bi is the physical slot (deterministic bucket first)
+          bi[k] = e == det_slot ? 0 : cc * D + d + 1;
           da[k] = s_a[e];
           dg[k] = s_g[e];
           if ((da[k] | (uint64_t)dg[k]) != 0) {
@@ -742,27 +748,28 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
   const bool valid = lane < g.D;
   for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * V; v0 < g.n; v0 += waves * V) {
-    uint64_t det_a[V], a[V];
-    G det_g[V], gm[V];
+    // physical slots 0..D of a row: lane 0 the deterministic bucket, lane d + 1 depth d of column 0
+    uint64_t ra[V];
+    G rg[V];
 #pragma unroll
     for (uint32_t k = 0; k < V; ++k) {
-      det_a[k] = 0;
-      det_g[k] = 0;
-      a[k] = 0;
-      gm[k] = 0;
-      if (v0 + k < g.n) {
-        ld_rec<G>(slab + rec_off<G>(g, v0 + k, g.nb - 1), &det_a[k], &det_g[k]);
-        if (valid) ld_rec<G>(slab + rec_off<G>(g, v0 + k, lane), &a[k], &gm[k]);
-      }
+      ra[k] = 0;
+      rg[k] = 0;
+      if (v0 + k < g.n && lane <= g.D)
+        ld_rec<G>(slab + (size_t)(v0 + k) * g.row + (size_t)lane * (8 + sizeof(G)), &ra[k], &rg[k]);
     }
 #pragma unroll
     for (uint32_t k = 0; k < V; ++k) {
       if (v0 + k >= g.n) break;
+      const uint64_t det_a = rd_lane_u64(ra[k], 0);
+      const G det_g = (G)rd_lane_u64((uint64_t)rg[k], 0);
+      const uint64_t a = shfl_u64(ra[k], (int)(lane + 1) & (kWave - 1));
+      const G gm = (G)shfl_u64((uint64_t)rg[k], (int)(lane + 1) & (kWave - 1));
       uint64_t idx;
-      const int code = wave_sample<G>(det_a[k], det_g[k], a[k], gm[k], valid, g.chk_bf, &idx);
+      const int code = wave_sample<G>(det_a, det_g, valid ? a : 0, valid ? gm : (G)0, valid, g.chk_bf, &idx);
       if (lane == 0) samp[v0 + k] = sample_record(code, idx, g.n, flags);
       // the deterministic bucket does not depend on the round: later rounds read this compact copy
-      if (lane == 1) st_rec<G>(det_cpy + (size_t)(v0 + k) * (8 + sizeof(G)), det_a[k], det_g[k]);
+      if (lane == 1) st_rec<G>(det_cpy + (size_t)(v0 + k) * (8 + sizeof(G)), det_a, det_g);
     }
   }
 }
