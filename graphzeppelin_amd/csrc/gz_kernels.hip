@@ -1127,14 +1127,12 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
   uint64_t acc_a = 0, acc_g = 0, all_a = 0, all_g = 0;
   uint32_t cur = first_root, seg_start = p0;
   constexpr uint32_t U = FIRST ? 8 : 16;  // loads in flight per lane
-  // Batches are software-pipelined: batch j + U is issued before batch j is reduced, so the
-  // segment handling and sampling of one batch overlap the memory latency of the next.
-  uint64_t va[U], vg[U];
-  auto load_batch = [&](uint32_t j, uint64_t* xa, uint64_t* xg) {
+  for (uint32_t j = 0; j < len; j += U) {
+    uint64_t va[U], vg[U];
 #pragma unroll
     for (uint32_t k = 0; k < U; ++k) {
-      xa[k] = 0;
-      xg[k] = 0;
+      va[k] = 0;
+      vg[k] = 0;
       if (j + k < len && loads) {
         if (FIRST) {
           // lanes < D read the column's records, lane D the compact deterministic bucket: one
@@ -1143,24 +1141,15 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
           const unsigned char* src = in_col ? slab + rec_off<G>(g, mv, boff)
                                             : reinterpret_cast<const unsigned char*>(det_cpy) + (size_t)mv * (8 + sizeof(G));
           G gw;
-          ld_rec<G>(src, &xa[k], &gw);
-          xg[k] = (uint64_t)gw;
+          ld_rec<G>(src, &va[k], &gw);
+          vg[k] = (uint64_t)gw;
         } else {  // empty slots hold zeros, so the values need not wait for the roots
           const uint64_t* e = in_part + (size_t)(p0 + j + k) * 2 * kWave;
-          xa[k] = e[lane];
-          xg[k] = e[kWave + lane];
+          va[k] = e[lane];
+          vg[k] = e[kWave + lane];
         }
       }
     }
-  };
-  // upper levels (one batch of 16 partials, LDS-fed slots) gain nothing from pipelining and
-  // would spill registers
-  constexpr bool kPipe = FIRST;
-  load_batch(0, va, vg);
-  for (uint32_t j = 0; j < len; j += U) {
-    uint64_t na[kPipe ? U : 1], ng[kPipe ? U : 1];
-    if (kPipe && j + U < len) load_batch(j + U, na, ng);
-    if (!kPipe && j > 0) load_batch(j, va, vg);
 #pragma unroll
     for (uint32_t k = 0; k < U; ++k) {
       if (j + k < len) {
@@ -1176,13 +1165,6 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
         acc_g ^= vg[k];
         all_a ^= va[k];
         all_g ^= vg[k];
-      }
-    }
-    if (kPipe) {
-#pragma unroll
-      for (uint32_t k = 0; k < (kPipe ? U : 1); ++k) {
-        va[k] = na[k];
-        vg[k] = ng[k];
       }
     }
   }
