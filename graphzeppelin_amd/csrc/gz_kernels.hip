@@ -198,15 +198,6 @@ __device__ inline bool sf_contains(const DsuDev& d, uint64_t key) {
   }
   return false;
 }
-// A successful merge records the edge in the spanning forest (cc_sketch_alg.cpp:152-162): in the
-// slot of the root it linked (every root is linked at most once between resets, so slots never
-// collide) and in the membership set used by the eager DSU.
-__device__ inline void sf_record(const DsuDev& d, uint32_t child, uint32_t u, uint32_t v) {
-  const uint32_t a = u < v ? u : v, b = u < v ? v : u;
-  const uint64_t key = ((uint64_t)a << 32) | b;
-  d.sf_edge[child] = key;
-  sf_insert(d, key);
-}
 
 // Sample records of round 0: a GOOD sample stores its edge index, whose endpoints are both below
 // n <= 2^32 - 1 after the range check, so it never equals the two sentinels (high half all ones)
@@ -737,6 +728,15 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
   for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
 }
 
+// Membership set of the spanning forest after a query (the eager DSU's deletion check): every
+// root linked by Borůvka holds its forest edge in sf_edge.
+__global__ void k_sf_build(DsuDev d) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += gridDim.x * blockDim.x) {
+    const uint64_t e = d.sf_edge[v];
+    if (e != kEmpty) sf_insert(d, e);
+  }
+}
+
 // Round 0 (run_round_zero, 223-244): every vertex samples column 0 of its own sketch.  One wave
 // per vertex; lane l checks depth l, the first good depth is the lowest set bit of the ballot.
 template <class G>
@@ -813,7 +813,11 @@ __device__ inline bool apply_sample(const DsuDev& d, uint64_t s) {
   const uint32_t a = (uint32_t)(s >> 32), b = (uint32_t)s;
   const uint32_t child = dsu_link(d.parent, a, b, d.prio_seed);
   if (child == kNoLink) return false;
-  sf_record(d, child, a, b);
+  // A merge records its edge in the spanning forest (cc_sketch_alg.cpp:152-162), in the slot of
+  // the root it linked (every root is linked at most once between resets, so slots never
+  // collide); the membership set the eager DSU uses is built once after the last round
+  // (k_sf_build), off the merge's critical path.
+  d.sf_edge[child] = a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a;
   return true;
 }
 
@@ -890,7 +894,6 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
     st[kStDone] = 0;
   }
   __syncthreads();
-  const uint32_t lane = lane_id();
   uint32_t r = 0;
   if (in_walk) {
     r = dsu_find_stable(parent, v);
@@ -900,14 +903,9 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
   // full path compression: every walked vertex now points at its root (no links in this kernel)
   if (in_walk) parent[v] = r;
   const bool active = in_walk && r != gcur;  // members of the giant are not counted
-  uint64_t todo = __ballot(active);
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const uint32_t lr = rd_lane_u32(r, leader);
-    const uint64_t same = __ballot(active && r == lr) & todo;
-    if ((int)lane == leader) atomicAdd(t_cnt + lds_table_slot(t_key, lr), (uint32_t)__popcll(same));
-    todo &= ~same;
-  }
+  // every lane inserts its root at once: LDS atomics absorb lanes that share a root, and no wave
+  // walks its distinct roots one by one
+  if (active) atomicAdd(t_cnt + lds_table_slot(t_key, r), 1u);
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
     if (t_key[k] != kEmptyKey) atomicAdd(cnt + t_key[k], t_cnt[k]);
@@ -972,27 +970,11 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
     t_cnt[k] = 0;
   }
   __syncthreads();
-  const uint32_t lane = lane_id();
-  const uint64_t lt = (1ull << lane) - 1;
   const bool active = in_walk && r != gr;  // the giant supernode's members are not regrouped
   uint32_t my_slot = 0, my_pos = 0;
-  uint64_t todo = __ballot(active);
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const uint32_t lr = rd_lane_u32(r, leader);
-    const uint64_t same = __ballot(active && r == lr) & todo;
-    uint32_t slot = 0, at = 0;
-    if ((int)lane == leader) {
-      slot = lds_table_slot(t_key, lr);
-      at = atomicAdd(t_cnt + slot, (uint32_t)__popcll(same));  // position of this group in the block's share
-    }
-    slot = __shfl(slot, leader);
-    at = __shfl(at, leader);
-    if ((same >> lane) & 1) {
-      my_slot = slot;
-      my_pos = at + (uint32_t)__popcll(same & lt);
-    }
-    todo &= ~same;
+  if (active) {  // every lane at once; the returned count is the lane's position in the block's share
+    my_slot = lds_table_slot(t_key, r);
+    my_pos = atomicAdd(t_cnt + my_slot, 1u);
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
@@ -1572,6 +1554,10 @@ void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const Ds
   hipLaunchKernelGGL(k_eager_union, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
   hipLaunchKernelGGL(k_eager_sf_insert, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
   hipLaunchKernelGGL(k_eager_sf_check, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
+}
+
+void launch_sf_build(const DsuDev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_sf_build, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, s, d);
 }
 
 void launch_dsu_reset(const DsuDev& d, hipStream_t s) {

@@ -104,6 +104,8 @@ size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
 void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s);
 // Reset the round state of a query (before round 1).
+// Insert every recorded forest edge (sf_edge) into the forest membership set (after a query).
+void launch_sf_build(const DsuDev& d, hipStream_t s);
 void launch_round_start(const RoundBufs& rb, hipStream_t s);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
 void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,

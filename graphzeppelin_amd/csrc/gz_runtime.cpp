@@ -681,6 +681,7 @@ void GzAlg::boruvka() {
       GZ_HIP(hipGetLastError());
       next = hi + 1;
     }
+    if (cfg.eager_dsu) launch_sf_build(dsu, st);  // the eager DSU's deletion check needs the set
     if (prof) {
       prof_end();
       timed.back().host_bytes = round_bytes * (double)(rounds + 1);
