@@ -68,13 +68,22 @@ struct Geo {
   uint32_t cps;      // cols_per_sample
   uint32_t samples;  // num_samples
   uint32_t nb;       // buckets per sketch = C*D + 1 (deterministic bucket last)
-  uint32_t row;      // bytes between consecutive sketches in the slab (nb records, 256-B aligned rows)
+  uint32_t row;      // bytes between consecutive sketches in the slab (row_bytes)
+  uint32_t cpitch;   // bytes between consecutive columns of a row (col_pitch_bytes)
   uint32_t Dp;       // LDS column pitch (D rounded up to 1 mod 16 to spread LDS banks)
   uint64_t seed;     // sketch seed (checksum seed)
   uint64_t chk_bf;   // bitflip of the checksum seed
 };
 
 inline uint32_t lds_pitch(uint32_t D) { return D + ((17u - (D % 16u)) % 16u); }
+
+// Row of one vertex sketch in the slab: column c's D records (depth d at byte c*cpitch + d*rb),
+// each column starting on a 128-B cache line; the deterministic bucket follows column 0's records
+// (byte D*rb), in that column's padding.  rb = bytes of one bucket record.
+inline uint32_t col_pitch_bytes(uint32_t D, uint32_t rb) { return ((D + 1) * rb + 127u) & ~127u; }
+inline uint64_t row_bytes(uint32_t C, uint32_t D, uint32_t rb) {
+  return ((uint64_t)C * col_pitch_bytes(D, rb) + 255u) & ~(uint64_t)255u;
+}
 
 // Per-tile work descriptor of the update kernel: `count` destinations of `vertex` starting at
 // `begin` in the destination array; `split` = the vertex has other tiles in the same launch, so

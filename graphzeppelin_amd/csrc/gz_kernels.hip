@@ -69,19 +69,25 @@ __device__ inline void st_rlx(T* p, T v) {
 }
 
 // ---- the slab: packed bucket records -----------------------------------------------------------
-// Bucket b of sketch v is the record {u64 alpha; G gamma} -- the reference's packed Bucket, 12 or 16
-// bytes -- at byte v * g.row + b * (8 + sizeof(G)).  The low depths of a column, which most
-// updates touch, are then adjacent in one cache line with their checksums.  A 12-byte record is
-// only 4-byte aligned, so it is moved as 32-bit words, and a 64-bit XOR atomic on it becomes two
-// 32-bit ones (XOR acts bit by bit, so the result is the same).
-// Physical order within a row: the deterministic bucket (reference index nb - 1) first, then the
-// columns in reference order, so the deterministic bucket and column 0 -- all that round 0 reads,
-// and the records every update touches most -- are one contiguous run.  rec_off takes the
-// reference bucket index.
-__host__ __device__ inline uint32_t phys_slot(const Geo& g, uint32_t b) { return b == g.nb - 1 ? 0 : b + 1; }
+// Every bucket is the record {u64 alpha; G gamma} -- the reference's packed Bucket, 12 or 16 bytes.
+// Row v of the slab holds sketch v.  Column c starts on a 128-B cache line (byte c * g.cpitch) and
+// holds depth d at c * g.cpitch + d * (8 + sizeof(G)), so the low depths of a column, which most
+// updates touch, share one line with their checksums.  The deterministic bucket follows column 0's
+// records, in that column's padding: the deterministic bucket and column 0 -- all that round 0
+// reads -- are one contiguous run, with the deterministic bucket in lane D of a wave that reads
+// it.  A 12-byte record is only 4-byte aligned, so it is moved as 32-bit words, and a 64-bit XOR
+// atomic on it becomes two 32-bit ones (XOR acts bit by bit, so the result is the same).
+// rec_col_off: byte of (column c, depth d) within a row; rec_off: byte of the reference's bucket
+// index b (c * D + d, the deterministic bucket last) of sketch v.
+template <class G>
+__device__ inline uint32_t rec_col_off(const Geo& g, uint32_t c, uint32_t d) {
+  return c * g.cpitch + d * (uint32_t)(8 + sizeof(G));
+}
 template <class G>
 __device__ inline size_t rec_off(const Geo& g, uint32_t v, uint32_t b) {
-  return (size_t)v * g.row + (size_t)phys_slot(g, b) * (8 + sizeof(G));
+  const uint32_t c = b == g.nb - 1 ? 0 : b / g.D;
+  const uint32_t d = b == g.nb - 1 ? g.D : b - c * g.D;
+  return (size_t)v * g.row + rec_col_off<G>(g, c, d);
 }
 template <class G>
 __device__ inline void ld_rec(const unsigned char* p, uint64_t* a, G* gm) {
@@ -273,7 +279,6 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   const uint32_t C = g.C, D = g.D;
   const uint32_t det_slot = C * kLdsPitch;
   const uint32_t lds_n = det_slot + 1;
-  constexpr uint32_t RB = 8 + sizeof(G);
   uint64_t* s_bf = reinterpret_cast<uint64_t*>(smem);
   uint64_t* s_a = s_bf + C;
   G* s_g = reinterpret_cast<G*>(s_a + lds_n);
@@ -340,7 +345,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
               lds_xor(s_a + li, idx);
               lds_xor(s_g + li, chk);
             } else {
-              xor_rec_atomic<G>(row + (size_t)(c * D + depth + 1) * RB, idx, chk);  // physical slot
+              xor_rec_atomic<G>(row + rec_col_off<G>(g, c, depth), idx, chk);
             }
           }
         }
@@ -372,18 +377,18 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     for (uint32_t e0 = 0; e0 < lds_n; e0 += 4 * kWave) {
       uint64_t da[4], ga[4];
       G dg[4], gg[4];
-      uint32_t bi[4];
+      uint32_t bo[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const uint32_t e = e0 + k * kWave + lane;
         da[k] = 0;
         dg[k] = 0;
-        bi[k] = 0;
+        bo[k] = 0;
         if (e < lds_n) {
           const uint32_t cc = e / kLdsPitch, d = e - cc * kLdsPitch;
           // entries at the pitch padding (d == kLdsDepth) or at depth >= D are never written;
-          // bi is the physical slot (deterministic bucket first)
-          bi[k] = e == det_slot ? 0 : cc * D + d + 1;
+          // bo is the record's byte offset in the row
+          bo[k] = e == det_slot ? rec_col_off<G>(g, 0, D) : rec_col_off<G>(g, cc, d);
           da[k] = s_a[e];
           dg[k] = s_g[e];
           if ((da[k] | (uint64_t)dg[k]) != 0) {
@@ -395,17 +400,17 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
       if (tl.split) {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          if ((da[k] | (uint64_t)dg[k]) != 0) xor_rec_atomic<G>(row + (size_t)bi[k] * RB, da[k], dg[k]);
+          if ((da[k] | (uint64_t)dg[k]) != 0) xor_rec_atomic<G>(row + bo[k], da[k], dg[k]);
       } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           ga[k] = 0;
           gg[k] = 0;
-          if ((da[k] | (uint64_t)dg[k]) != 0) ld_rec<G>(row + (size_t)bi[k] * RB, &ga[k], &gg[k]);
+          if ((da[k] | (uint64_t)dg[k]) != 0) ld_rec<G>(row + bo[k], &ga[k], &gg[k]);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          if ((da[k] | (uint64_t)dg[k]) != 0) st_rec<G>(row + (size_t)bi[k] * RB, ga[k] ^ da[k], (G)(gg[k] ^ dg[k]));
+          if ((da[k] | (uint64_t)dg[k]) != 0) st_rec<G>(row + bo[k], ga[k] ^ da[k], (G)(gg[k] ^ dg[k]));
       }
     }
     __syncthreads();
@@ -748,7 +753,7 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
   const bool valid = lane < g.D;
   for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * V; v0 < g.n; v0 += waves * V) {
-    // physical slots 0..D of a row: lane 0 the deterministic bucket, lane d + 1 depth d of column 0
+    // records 0..D of a row: lane d < D depth d of column 0, lane D the deterministic bucket
     uint64_t ra[V];
     G rg[V];
 #pragma unroll
@@ -761,15 +766,13 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
 #pragma unroll
     for (uint32_t k = 0; k < V; ++k) {
       if (v0 + k >= g.n) break;
-      const uint64_t det_a = rd_lane_u64(ra[k], 0);
-      const G det_g = (G)rd_lane_u64((uint64_t)rg[k], 0);
-      const uint64_t a = shfl_u64(ra[k], (int)(lane + 1) & (kWave - 1));
-      const G gm = (G)shfl_u64((uint64_t)rg[k], (int)(lane + 1) & (kWave - 1));
+      const uint64_t det_a = rd_lane_u64(ra[k], g.D);
+      const G det_g = (G)rd_lane_u64((uint64_t)rg[k], g.D);
       uint64_t idx;
-      const int code = wave_sample<G>(det_a, det_g, valid ? a : 0, valid ? gm : (G)0, valid, g.chk_bf, &idx);
+      const int code = wave_sample<G>(det_a, det_g, valid ? ra[k] : 0, valid ? rg[k] : (G)0, valid, g.chk_bf, &idx);
       if (lane == 0) samp[v0 + k] = sample_record(code, idx, g.n, flags);
       // the deterministic bucket does not depend on the round: later rounds read this compact copy
-      if (lane == 1) st_rec<G>(det_cpy + (size_t)(v0 + k) * (8 + sizeof(G)), det_a, det_g);
+      if (lane == g.D) st_rec<G>(det_cpy + (size_t)(v0 + k) * (8 + sizeof(G)), ra[k], rg[k]);
     }
   }
 }
@@ -1068,7 +1071,7 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
   const uint32_t p1 = p0 + len;
 
   const bool in_col = lane < g.D;
-  const uint32_t boff = in_col ? round * g.D + lane : (g.nb - 1);
+  const uint32_t coff = in_col ? rec_col_off<G>(g, round, lane) : 0;  // depth `lane` of column `round`
   const bool loads = lane <= g.D;
   bool wrote_head = false, wrote_tail = false;
 
@@ -1120,7 +1123,7 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
           // lanes < D read the column's records, lane D the compact deterministic bucket: one
           // load instruction with per-lane addresses
           const uint32_t mv = rd_lane_u32(m, j + k);
-          const unsigned char* src = in_col ? slab + rec_off<G>(g, mv, boff)
+          const unsigned char* src = in_col ? slab + (size_t)mv * g.row + coff
                                             : reinterpret_cast<const unsigned char*>(det_cpy) + (size_t)mv * (8 + sizeof(G));
           G gw;
           ld_rec<G>(src, &va[k], &gw);

@@ -41,7 +41,8 @@ static Geo make_geo(uint32_t nsk, uint64_t vector_len, uint64_t seed, size_t sam
   g.cps = (uint32_t)cols;
   g.C = (uint32_t)(samples * cols);
   g.nb = g.C * g.D + 1;
-  g.row = (uint32_t)(((size_t)g.nb * (8 + gb) + 255) & ~(size_t)255);
+  g.cpitch = col_pitch_bytes(g.D, 8 + gb);
+  g.row = (uint32_t)row_bytes(g.C, g.D, 8 + gb);
   g.Dp = lds_pitch(g.D);
   g.seed = seed;
   g.chk_bf = xxh3_bitflip(seed);
@@ -163,7 +164,7 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const size_t D = calc_bkt_per_col(len), C = calc_cc_samples(n, c.sketches_factor);
   const uint64_t nb = C * D + 1;
   const int gbytes = c.gamma_bytes ? c.gamma_bytes : 4;
-  const uint64_t slab = (uint64_t)n * ((nb * (8 + gbytes) + 255) & ~255ull);
+  const uint64_t slab = (uint64_t)n * row_bytes((uint32_t)C, (uint32_t)D, 8 + gbytes);
   const size_t desired = (size_t)(nb * (8 + gbytes) / 4 * c.batch_factor);
   const uint64_t cap = c.flush_threshold ? c.flush_threshold : default_cap_pairs(n, desired);
   const uint32_t t = c.tile ? c.tile : 1024;
