@@ -224,18 +224,20 @@ __device__ inline uint64_t sample_record(int code, uint64_t idx, uint32_t n, uin
 }
 
 // ---- sample (src/sketch.cpp:94-116) over one wave --------------------------------------------
-// Lane l < D holds bucket (column, depth l); every lane holds the deterministic bucket.
+// Lane l < D holds bucket (column, depth l), lane D the deterministic bucket.  Every lane checks its
+// own record with one hash, so the deterministic bucket's check runs alongside the column's.
 // Returns the code; *idx gets the sampled index on GOOD.
 template <class G>
-__device__ inline int wave_sample(uint64_t det_a, G det_g, uint64_t a, G gm, bool lane_valid, uint64_t chk_bf,
-                                  uint64_t* idx) {
+__device__ inline int wave_sample(uint64_t a, G gm, uint32_t D, uint64_t chk_bf, uint64_t* idx) {
+  const uint32_t lane = lane_id();
+  const uint64_t mask = __ballot(lane <= D && (G)xxh3_mix(xxh3_prep(a), chk_bf) == gm);
+  const uint64_t det_a = rd_lane_u64(a, (int)D);
+  const G det_g = (G)rd_lane_u64((uint64_t)gm, (int)D);
   if (det_a == 0 && det_g == 0) { *idx = 0; return kZero; }
-  if ((G)xxh3_mix(xxh3_prep(det_a), chk_bf) == det_g) { *idx = det_a; return kGood; }
-  bool good = lane_valid && (G)xxh3_mix(xxh3_prep(a), chk_bf) == gm;
-  uint64_t mask = __ballot(good);
-  if (mask) {
-    int first = __builtin_ctzll(mask);
-    *idx = rd_lane_u64(a, first);
+  if ((mask >> D) & 1) { *idx = det_a; return kGood; }
+  const uint64_t col = mask & ((1ull << D) - 1);
+  if (col) {
+    *idx = rd_lane_u64(a, __builtin_ctzll(col));
     return kGood;
   }
   *idx = 0;
@@ -751,7 +753,6 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
   constexpr uint32_t V = 4;  // vertices per wave iteration: all their loads are issued first
   const uint32_t lane = lane_id();
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
-  const bool valid = lane < g.D;
   for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * V; v0 < g.n; v0 += waves * V) {
     // records 0..D of a row: lane d < D depth d of column 0, lane D the deterministic bucket
     uint64_t ra[V];
@@ -766,10 +767,8 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
 #pragma unroll
     for (uint32_t k = 0; k < V; ++k) {
       if (v0 + k >= g.n) break;
-      const uint64_t det_a = rd_lane_u64(ra[k], g.D);
-      const G det_g = (G)rd_lane_u64((uint64_t)rg[k], g.D);
       uint64_t idx;
-      const int code = wave_sample<G>(det_a, det_g, valid ? ra[k] : 0, valid ? rg[k] : (G)0, valid, g.chk_bf, &idx);
+      const int code = wave_sample<G>(ra[k], rg[k], g.D, g.chk_bf, &idx);
       if (lane == 0) samp[v0 + k] = sample_record(code, idx, g.n, flags);
       // the deterministic bucket does not depend on the round: later rounds read this compact copy
       if (lane == g.D) st_rec<G>(det_cpy + (size_t)(v0 + k) * (8 + sizeof(G)), ra[k], rg[k]);
@@ -797,8 +796,7 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot,
 //                     vertices of (count << 32 | root); the next round excludes that supernode.
 //                     The complement sum needs no majority -- any supernode's sum is the XOR of the
 //                     vertices outside it -- so a large one can be left out a round early.
-//   kStDone           workgroups of this round's upper-level reduce that have finished level 2
-constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStDone = 5, kStBest = 6;
+constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStBest = 6;
 constexpr uint32_t kStWords = 8;
 constexpr uint32_t kCandidateDiv = 32;
 constexpr uint32_t kMaxStatRounds = 64;
@@ -894,7 +892,6 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
     st[kStGiantRound] = gcur;
     st[kStGiantFirst] = gfirst;
     st[kStActMode] = act ? 1u : 0u;
-    st[kStDone] = 0;
   }
   __syncthreads();
   uint32_t r = 0;
@@ -986,77 +983,66 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
   if (active) members[t_base[my_slot] + my_pos] = v;
 }
 
-// Segmented XOR-reduce + sample, one level (range_merge of column `round` + the deterministic
-// bucket over every member of a supernode, then sample_supernode).  Entries are grouped by root.
-// A wave takes 64 consecutive entries; lane l < D accumulates depth l of the column and lane D the
-// deterministic bucket.  A supernode whose entries all lie inside the range is sampled from
+// Segmented XOR-reduce + sample (range_merge of column `round` + the deterministic bucket over
+// every member of a supernode, then sample_supernode).  Entries are the member list, grouped by
+// root.  A wave takes 64 consecutive entries; lane l < D accumulates depth l of the column and lane
+// D the deterministic bucket.  A supernode whose entries all lie inside the range is sampled from
 // registers; its sample record goes to samp[root] for k_apply_samples.  A supernode that crosses
 // the range's first (head) or last (tail) boundary leaves its partial XOR in the range's head or
-// tail slot under its root; a slot left unused gets root kNoRoot (its value is never read).
+// tail slot under its root; a slot left unused gets root kNoRoot and a zero value.
 //
 // A workgroup of 16 waves covers 1024 entries: the waves leave their slots in LDS, then one wave
-// reduces those 32 slots the same way, so a workgroup emits at most two partials (plus one
-// total, below) per 1024 entries.  The next level reduces the partials of all workgroups, so a
-// supernode of any size is combined without atomics, and at n = 2^18 the second level fits in one
-// workgroup.
+// reduces those 32 slots the same way, so at most two supernodes of a workgroup are unfinished:
+// the one continuing from the previous workgroup and the one continuing into the next.
+//
+// Cross-workgroup supernodes.  A supernode whose entries span workgroups f..l (f < l) is combined
+// with device-scope XOR atomics into the accumulator slot of f, its first workgroup (only one
+// supernode per workgroup can continue past its end, so f identifies it).  Every workgroup of f..l
+// adds its part once and then counts itself in slot f's counter; the one that brings the count to
+// l - f + 1 samples the sum and clears the slot and the counter for the next round.  The segment
+// bounds come from k_bv_alloc / k_bv_scatter (off = start, cursor = end).  No workgroup waits for
+// another, and the XOR is order-free, so the sum is exactly the member-wise merge.
 //
 // Giant supernode.  Every edge update XORs the same buckets into both endpoints' sketches, so the
 // XOR of all vertex sketches is zero and a supernode's sum equals the XOR of every vertex outside
 // it.  When a supernode is summed from its complement (the sketches known to sum to zero) its
-// members are not regrouped: each level-1 workgroup also XORs all of its entries into a total,
-// the totals form one extra segment [2B, 3B) under the giant's root at level 2, and the giant is
-// sampled from that sum -- the same bytes the member-wise merge would give.
-//
-// Entry counts are on the device: level 1 has e1 = (entries regrouped) entries; level 2 has
-// 2*B1 (+ B1 totals with a giant) with B1 = ceil(e1/1024); later levels 2*ceil(e/1024).  A level
-// whose predecessor had a single workgroup is a no-op.
+// members are not regrouped: each workgroup XORs all of its entries into a total and adds it to
+// the giant's accumulator slot (slot gridDim.x), counted like the others, and the last workgroup
+// samples the giant from that sum -- the same bytes the member-wise merge would give.
 constexpr uint32_t kRedWaves = 16, kRedBlock = kRedWaves * kWave;
-// entries per wave: 64 rows at level 1; 16 partials above, one batch of loads, so the small upper
-// levels spread over more waves instead of chaining dependent batches in a few
-constexpr uint32_t kUpPerWave = 16;
-__host__ __device__ inline uint32_t level_span(uint32_t level) {
-  return kRedWaves * (level == 1 ? kWave : kUpPerWave);
-}
-
-__device__ inline bool level_entries(uint32_t e1, bool giant, uint32_t level, uint32_t* e) {
-  uint32_t cur = e1;
-  for (uint32_t l = 1; l < level; ++l) {
-    const uint32_t span = level_span(l);
-    const uint32_t b = (cur + span - 1) / span;
-    if (b <= 1) return false;
-    cur = 2 * b + ((l == 1 && giant) ? b : 0);
-  }
-  *e = cur;
-  return true;
-}
 
 // Slot of a range's boundary partial: 128 u64 (alpha of lanes 0..63, then gamma) and a root.
-constexpr uint32_t kNeighborLoad = 0xFFFFFFFEu;  // reduce_range: load the neighbour root itself
 struct Slot {
   uint64_t* vals;
   uint32_t* root;
 };
 
+// Cross-workgroup accumulators: (gridDim.x + 1) slots of 128 u64 and as many arrival counters, all
+// zero between rounds.
+struct XAcc {
+  uint64_t* vals;
+  uint32_t* cnt;
+  const uint32_t* seg_off;  // segment start per root
+  const uint32_t* seg_end;  // segment end per root (the scatter cursor after k_bv_scatter)
+};
+
 // Reduce entries [p0, p0 + len) (len <= 64) with one wave.  prev_root / next_root: roots of the
 // entries just before and after the range (kNoRoot when none), which decide whether the first and
 // last segments are open.  FIRST: entries are members (rows of the slab); otherwise partial
-// entries (in_part / in_root, global or LDS).  *all_a / *all_g get the XOR of every entry read.
+// entries (in_part / in_root in LDS).  *all_a / *all_g get the XOR of every entry read.  FIRST
+// with s_bnd: the wave holding the workgroup's first entry [blk_b0] loads that entry's segment
+// bounds into s_bnd[0..1], the one holding its last entry the end of that entry's segment into
+// s_bnd[2], in flight with the values.
 template <class G, bool FIRST>
-__device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_root_in, uint32_t next_root_in,
+__device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_root, uint32_t next_root,
                                     const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
                                     const unsigned char* __restrict__ slab, const Geo& g, uint32_t round,
                                     const uint64_t* in_part, const uint32_t* in_root, Slot head, Slot tail,
                                     uint64_t* __restrict__ samp, uint32_t* __restrict__ flags, uint64_t* all_a_out,
-                                    uint64_t* all_g_out, const unsigned char* __restrict__ det_cpy) {
+                                    uint64_t* all_g_out, const unsigned char* __restrict__ det_cpy,
+                                    const XAcc* xa, uint32_t blk_b0, uint32_t blk_b1, uint32_t* s_bnd) {
   const uint32_t lane = lane_id();
   uint32_t m = 0, rt = kNoRoot;
-  // prev_root_in / next_root_in == kNeighborLoad: the neighbours are entries p0 - 1 and p0 + len of
-  // in_root, loaded together with the range's roots (no extra round trip)
-  uint32_t prev_root = prev_root_in, next_root = next_root_in;
-  if (!FIRST) {
-    if (prev_root_in == kNeighborLoad) prev_root = in_root[p0 - 1];
-    if (next_root_in == kNeighborLoad) next_root = in_root[p0 + len];
-  }
   if (lane < len) {
     if (FIRST) {
       m = members[p0 + lane];
@@ -1069,6 +1055,12 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
   const bool head_open = first_root != kNoRoot && prev_root == first_root;
   const bool tail_open = last_root != kNoRoot && next_root == last_root;
   const uint32_t p1 = p0 + len;
+  const bool holds_first = FIRST && s_bnd && p0 == blk_b0;
+  const bool holds_last = FIRST && s_bnd && p1 == blk_b1;
+  uint32_t bnd = 0;
+  if (holds_first && lane == 0) bnd = xa->seg_off[first_root];
+  if (holds_first && lane == 1) bnd = xa->seg_end[first_root];
+  if (holds_last && lane == 2) bnd = xa->seg_end[last_root];
 
   const bool in_col = lane < g.D;
   const uint32_t coff = in_col ? rec_col_off<G>(g, round, lane) : 0;  // depth `lane` of column `round`
@@ -1085,17 +1077,15 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
     const bool continues = seg_end == p1 && tail_open;
     if (!started_before && !continues) {
       if (R != kNoRoot) {
-        const uint64_t det_a = rd_lane_u64(acc_a, g.D);
-        const G det_g = (G)rd_lane_u64(acc_g, g.D);
         uint64_t idx;
-        const int code = wave_sample<G>(det_a, det_g, acc_a, (G)acc_g, in_col, g.chk_bf, &idx);
+        const int code = wave_sample<G>(acc_a, (G)acc_g, g.D, g.chk_bf, &idx);
         if (lane == 0) samp[R] = sample_record(code, idx, g.n, flags);
       }
       return;
     }
     // Partial of a spanning supernode: the head slot if it started before this range, else the
     // tail slot.  When it covers the whole range the tail slot carries the same root with a zero
-    // value, so the supernode's entries stay contiguous at the next level.
+    // value, so the supernode's entries stay contiguous in the next reduce.
     if (started_before) {
       put(head, R, acc_a, acc_g);
       wrote_head = true;
@@ -1154,48 +1144,80 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
     }
   }
   emit(cur, acc_a, acc_g, seg_start, p1);
-  // an unused slot: no root, zero value (the next level reads values without looking at roots)
+  // an unused slot: no root, zero value (the next reduce reads values without looking at roots)
   if (!wrote_head) put(head, kNoRoot, 0, 0);
   if (!wrote_tail) put(tail, kNoRoot, 0, 0);
+  if (holds_first && lane < 2) s_bnd[lane] = bnd;
+  if (holds_last && lane == 2) s_bnd[2] = bnd;
   *all_a_out = all_a;
   *all_g_out = all_g;
 }
 
-// One workgroup of kRedBlock threads reduces entries [1024*blk, 1024*blk + 1024) of n_entries into
-// the global slots 2*blk (head) and 2*blk+1 (tail) of out_part / out_root, plus (FIRST with a
-// giant) its total in slot 2*nblk + blk, or the giant's sample when the level has one workgroup.
-// Uniform over the workgroup; contains __syncthreads.
-template <class G, bool FIRST>
-__device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t giant, const uint32_t* members,
-                                    const uint32_t* vroot, const unsigned char* slab, const Geo& g, uint32_t round,
-                                    const uint64_t* in_part, const uint32_t* in_root, uint64_t* out_part,
-                                    uint32_t* out_root, uint64_t* samp, uint32_t* flags, uint64_t* s_vals,
-                                    uint32_t* s_root, uint64_t* s_tot, const unsigned char* det_cpy) {
-  constexpr uint32_t per_wave = FIRST ? kWave : kUpPerWave;
-  constexpr uint32_t span = kRedWaves * per_wave;
+// Per lane: XOR (a, gm) into an accumulator slot with device-scope atomics (zero words skipped).
+__device__ inline void xacc_add(uint64_t* slot, uint64_t a, uint64_t gm) {
+  const uint32_t lane = lane_id();
+  if (a) __hip_atomic_fetch_xor(slot + lane, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (gm) __hip_atomic_fetch_xor(slot + kWave + lane, gm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LDS of one reduce workgroup: 32 slots of 128 u64 and their roots, the giant total, the
+// workgroup's two unfinished supernodes and the segment bounds of its first and last entries.
+#define GZ_REDUCE_LDS                                             \
+  __shared__ uint64_t s_vals[2 * kRedWaves * 2 * kWave];          \
+  __shared__ uint32_t s_root[2 * kRedWaves];                      \
+  __shared__ uint64_t s_tot[2 * kWave];                           \
+  __shared__ uint64_t s_out[2 * 2 * kWave];                       \
+  __shared__ uint32_t s_oroot[2];                                 \
+  __shared__ uint32_t s_bnd[3];
+
+// The reduce of a whole round: one workgroup per 1024 entries of the member list (e1 entries,
+// written by k_bv_scatter), gridDim.x >= ceil(e1 / 1024).
+template <class G>
+__global__ void __launch_bounds__(kRedBlock)
+k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
+            const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
+            XAcc xa, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
+            const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats,
+            const unsigned char* __restrict__ det_cpy) {
+  const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
+  GZ_REDUCE_LDS
+  const uint32_t giant = st[kStGiantRound];
+  const uint32_t e1 = st[kStSeg + (round & 1)];
+  if (!live) return;
+  if (rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
+    rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
+    rstats[2 * round + 1] = giant;
+  }
+  if (giant != kNoRoot && e1 == 0) {  // every vertex is in the giant: its sum is zero
+    if (blockIdx.x == 0 && threadIdx.x == 0) samp[giant] = kSampZero;
+    return;
+  }
+  constexpr uint32_t span = kRedWaves * kWave;
+  const uint32_t blk = blockIdx.x;
+  if (blk * span >= e1) return;  // uniform over the workgroup
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
-  const uint32_t nblk = (n_entries + span - 1) / span;
+  const uint32_t nblk = (e1 + span - 1) / span;
   const uint32_t b0 = blk * span;
-  const uint32_t b1 = b0 + span < n_entries ? b0 + span : n_entries;
-  auto root_at = [&](uint32_t p) -> uint32_t { return FIRST ? vroot[members[p]] : in_root[p]; };
-  const bool giant_tot = FIRST && giant != kNoRoot;
+  const uint32_t b1 = b0 + span < e1 ? b0 + span : e1;
+  auto root_at = [&](uint32_t p) -> uint32_t { return vroot[members[p]]; };
+  const bool giant_tot = giant != kNoRoot;
   // neighbours of the workgroup's range, for phase 2; loaded early so they are in flight with phase 1
   const uint32_t blk_prev = (wid == 0 && b0 > 0) ? root_at(b0 - 1) : kNoRoot;
-  const uint32_t blk_next = (wid == 0 && b1 < n_entries) ? root_at(b1) : kNoRoot;
+  const uint32_t blk_next = (wid == 0 && b1 < e1) ? root_at(b1) : kNoRoot;
   if (giant_tot)
     for (uint32_t k = threadIdx.x; k < 2 * kWave; k += blockDim.x) s_tot[k] = 0;
   __syncthreads();
   // phase 1: every wave reduces its entries, leaving its boundary partials in LDS slots
-  const uint32_t p0 = b0 + wid * per_wave;
+  const uint32_t p0 = b0 + wid * kWave;
   Slot hs{s_vals + (size_t)(2 * wid) * 2 * kWave, s_root + 2 * wid};
   Slot ts{s_vals + (size_t)(2 * wid + 1) * 2 * kWave, s_root + 2 * wid + 1};
   if (p0 < b1) {
-    const uint32_t len = b1 - p0 < per_wave ? b1 - p0 : per_wave;
-    const uint32_t prev = p0 == 0 ? kNoRoot : (FIRST ? root_at(p0 - 1) : kNeighborLoad);
-    const uint32_t next = p0 + len >= n_entries ? kNoRoot : (FIRST ? root_at(p0 + len) : kNeighborLoad);
+    const uint32_t len = b1 - p0 < kWave ? b1 - p0 : kWave;
+    const uint32_t prev = p0 == 0 ? kNoRoot : root_at(p0 - 1);
+    const uint32_t next = p0 + len >= e1 ? kNoRoot : root_at(p0 + len);
     uint64_t wa, wg;
-    reduce_range<G, FIRST>(p0, len, prev, next, members, vroot, slab, g, round, in_part, in_root, hs, ts, samp, flags,
-                           &wa, &wg, det_cpy);
+    reduce_range<G, true>(p0, len, prev, next, members, vroot, slab, g, round, nullptr, nullptr, hs, ts, samp, flags,
+                          &wa, &wg, det_cpy, &xa, b0, b1, s_bnd);
     if (giant_tot) {
       lds_xor(s_tot + lane, wa);
       lds_xor(s_tot + kWave + lane, wg);
@@ -1211,109 +1233,53 @@ __device__ inline void reduce_block(uint32_t blk, uint32_t n_entries, uint32_t g
     }
   }
   __syncthreads();
-  // phase 2: wave 0 reduces the workgroup's 32 slots into its two global slots
-  if (wid == 0) {
-    const uint32_t prev = blk_prev, next = blk_next;
-    Slot gh{out_part + (size_t)(2 * blk) * 2 * kWave, out_root + 2 * blk};
-    Slot gt{out_part + (size_t)(2 * blk + 1) * 2 * kWave, out_root + 2 * blk + 1};
-    uint64_t wa, wg;
-    reduce_range<G, false>(0, 2 * kRedWaves, prev, next, nullptr, nullptr, nullptr, g, round, s_vals, s_root, gh, gt,
-                           samp, flags, &wa, &wg, nullptr);
+  if (wid != 0) return;  // no barrier follows
+  // phase 2: wave 0 reduces the workgroup's 32 slots; the supernodes continuing past either end of
+  // the range are left in s_out
+  Slot oh{s_out, s_oroot}, ot{s_out + 2 * kWave, s_oroot + 1};
+  uint64_t wa, wg;
+  reduce_range<G, false>(0, 2 * kRedWaves, blk_prev, blk_next, nullptr, nullptr, nullptr, g, round, s_vals, s_root, oh,
+                         ot, samp, flags, &wa, &wg, nullptr, nullptr, 0, 0, nullptr);
+  if (nblk == 1) {  // one workgroup: nothing continues, and its total is the giant's whole sum
     if (giant_tot) {
-      const uint64_t ta = s_tot[lane], tg = s_tot[kWave + lane];
-      if (nblk == 1) {  // one workgroup: its total is the giant's whole sum
-        const bool in_col = lane < g.D;
-        const uint64_t det_a = rd_lane_u64(ta, g.D);
-        const G det_g = (G)rd_lane_u64(tg, g.D);
-        uint64_t idx;
-        const int code = wave_sample<G>(det_a, det_g, ta, (G)tg, in_col, g.chk_bf, &idx);
-        if (lane == 0) samp[giant] = sample_record(code, idx, g.n, flags);
-      } else {
-        Slot tt{out_part + (size_t)(2 * nblk + blk) * 2 * kWave, out_root + 2 * nblk + blk};
-        tt.vals[lane] = ta;
-        tt.vals[kWave + lane] = tg;
-        if (lane == 0) *tt.root = giant;
-      }
+      uint64_t idx;
+      const int code = wave_sample<G>(s_tot[lane], (G)s_tot[kWave + lane], g.D, g.chk_bf, &idx);
+      if (lane == 0) samp[giant] = sample_record(code, idx, g.n, flags);
     }
-  }
-  __syncthreads();  // the LDS slots are reused by the caller's next call
-}
-
-// LDS of one reduce workgroup: 32 slots of 128 u64, their roots, the giant total.
-#define GZ_REDUCE_LDS                                             \
-  __shared__ uint64_t s_vals[2 * kRedWaves * 2 * kWave];          \
-  __shared__ uint32_t s_root[2 * kRedWaves];                      \
-  __shared__ uint64_t s_tot[2 * kWave];
-
-template <class G>
-__global__ void __launch_bounds__(kRedBlock)
-k_bv_reduce_first(const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
-                  const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
-                  uint64_t* __restrict__ out_part, uint32_t* __restrict__ out_root, uint64_t* __restrict__ samp,
-                  uint32_t* __restrict__ flags, const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats,
-                  const unsigned char* __restrict__ det_cpy) {
-  const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
-  GZ_REDUCE_LDS
-  const uint32_t giant = st[kStGiantRound];
-  const uint32_t e1 = st[kStSeg + (round & 1)];
-  if (!live) return;
-  if (rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
-    rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
-    rstats[2 * round + 1] = giant;
-  }
-  if (giant != kNoRoot && e1 == 0) {  // every vertex is in the giant: its sum is zero
-    if (blockIdx.x == 0 && threadIdx.x == 0) samp[giant] = kSampZero;
     return;
   }
-  if (blockIdx.x * level_span(1) >= e1) return;  // uniform over the workgroup
-  reduce_block<G, true>(blockIdx.x, e1, giant, members, vroot, slab, g, round, nullptr, nullptr, out_part, out_root,
-                        samp, flags, s_vals, s_root, s_tot, det_cpy);
-}
-
-// Levels >= 2 of the reduce.  Every workgroup takes its 1024 entries of level `level`.  With
-// `tail` (the host sets it once the following levels hold at most kTailEntries entries), the last
-// workgroup to finish -- a counter per round with device-scope fences around it, so no workgroup
-// ever waits -- then runs the remaining levels alone.  Level l reads part[l & 1] and writes
-// part[(l - 1) & 1].
-constexpr uint32_t kTailEntries = 4 * kRedWaves * kUpPerWave;
-template <class G>
-__global__ void __launch_bounds__(kRedBlock)
-k_bv_reduce_up(uint32_t level0, uint32_t tail, uint32_t* __restrict__ st, const Geo g, uint32_t round,
-               uint64_t* __restrict__ part0, uint64_t* __restrict__ part1, uint32_t* __restrict__ proot0,
-               uint32_t* __restrict__ proot1, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
-               const uint32_t* __restrict__ prev_mod) {
-  const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
-  GZ_REDUCE_LDS
-  const uint32_t giant = st[kStGiantRound];
-  const bool has_giant = giant != kNoRoot;
-  const uint32_t e1 = st[kStSeg + (round & 1)];
-  if (!live) return;
-  if (has_giant && e1 == 0) return;
-  uint32_t ne;
-  if (!level_entries(e1, has_giant, level0, &ne)) return;  // uniform over the grid: an earlier level was the last
-  uint64_t* part[2] = {part0, part1};
-  uint32_t* proot[2] = {proot0, proot1};
-  if (blockIdx.x * level_span(level0) < ne)
-    reduce_block<G, false>(blockIdx.x, ne, giant, nullptr, nullptr, nullptr, g, round, part[level0 & 1],
-                           proot[level0 & 1], part[(level0 - 1) & 1], proot[(level0 - 1) & 1], samp, flags, s_vals,
-                           s_root, s_tot, nullptr);
-  if (!tail) return;
-  __shared__ uint32_t s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = atomicAdd(st + kStDone, 1u) == gridDim.x - 1 ? 1u : 0u;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (uint32_t level = level0 + 1;; ++level) {
-    if (!level_entries(e1, has_giant, level, &ne)) break;
-    const uint32_t nb = (ne + level_span(level) - 1) / level_span(level);
-    for (uint32_t blk = 0; blk < nb; ++blk)
-      reduce_block<G, false>(blk, ne, giant, nullptr, nullptr, nullptr, g, round, part[level & 1], proot[level & 1],
-                             part[(level - 1) & 1], proot[(level - 1) & 1], samp, flags, s_vals, s_root, s_tot,
-                             nullptr);
-    __threadfence_block();
-    __syncthreads();
+  // Cross-workgroup parts: the head supernode started in workgroup fh (its segment start), the
+  // tail one starts here; each ends in the workgroup holding its segment's last entry.
+  const uint32_t hr = s_oroot[0], tr = s_oroot[1];
+  const bool has_h = hr != kNoRoot;
+  const bool has_t = tr != kNoRoot && tr != hr;  // a supernode covering the whole range is counted once
+  const uint32_t fh = has_h ? s_bnd[0] / span : 0, lh = has_h ? (s_bnd[1] - 1) / span : 0;
+  const uint32_t lt = has_t ? (s_bnd[2] - 1) / span : 0;
+  const uint32_t gslot = gridDim.x;
+  if (has_h) xacc_add(xa.vals + (size_t)fh * 2 * kWave, s_out[lane], s_out[kWave + lane]);
+  if (has_t) xacc_add(xa.vals + (size_t)blk * 2 * kWave, s_out[2 * kWave + lane], s_out[3 * kWave + lane]);
+  if (giant_tot) xacc_add(xa.vals + (size_t)gslot * 2 * kWave, s_tot[lane], s_tot[kWave + lane]);
+  __threadfence();  // the parts are visible before the arrival is counted
+  bool last = false;
+  if (lane == 0 && has_h) last = atomicAdd(xa.cnt + fh, 1u) == lh - fh;
+  if (lane == 1 && has_t) last = atomicAdd(xa.cnt + blk, 1u) == lt - blk;
+  if (lane == 2 && giant_tot) last = atomicAdd(xa.cnt + gslot, 1u) == nblk - 1;
+  const uint64_t wins = __ballot(last);
+  if (!wins) return;
+  __threadfence();  // every part counted before ours is visible
+  for (uint32_t k = 0; k < 3; ++k) {
+    if (!((wins >> k) & 1)) continue;
+    const uint32_t slot = k == 0 ? fh : k == 1 ? blk : gslot;
+    const uint32_t R = k == 0 ? hr : k == 1 ? tr : giant;
+    uint64_t* sv = xa.vals + (size_t)slot * 2 * kWave;
+    const uint64_t a = ld_acq(sv + lane), gm = ld_acq(sv + kWave + lane);
+    uint64_t idx;
+    const int code = wave_sample<G>(a, (G)gm, g.D, g.chk_bf, &idx);
+    if (lane == 0) samp[R] = sample_record(code, idx, g.n, flags);
+    // cleared for the next round: every other part of this slot has been counted
+    st_rlx(sv + lane, (uint64_t)0);
+    st_rlx(sv + kWave + lane, (uint64_t)0);
+    if (lane == 0) st_rlx(xa.cnt + slot, 0u);
   }
 }
 
@@ -1584,23 +1550,7 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
 // Host bound on each level's entry count (the kernels compute the exact counts on the device):
 // the larger of the chain without a supernode summed from its complement (level 2 = 2 * B1) and
 // with one (level 2 = 3 * B1); e1 <= n in both.
-size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels) {
-  uint32_t a = n, b = n;
-  size_t L = 0;
-  bool a_done = false, b_done = false;
-  while (true) {
-    if (L < max_levels && entries_per_level) entries_per_level[L] = std::max(a_done ? 0u : a, b_done ? 0u : b);
-    ++L;
-    const uint32_t span = level_span((uint32_t)L);
-    const uint32_t wa = (a + span - 1) / span, wb = (b + span - 1) / span;
-    if (wa <= 1) a_done = true;
-    if (wb <= 1) b_done = true;
-    if (a_done && b_done) break;
-    a = 2 * wa;
-    b = 2 * wb + (L == 1 ? wb : 0);
-  }
-  return L;
-}
+uint32_t reduce_blocks(uint32_t n) { return std::max(1u, (n + kRedBlock - 1) / kRedBlock); }
 
 void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s) {
@@ -1617,29 +1567,14 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
                      giant_ok ? 1u : 0u, prev);
   hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, st, round,
                      act_list, prev);
-  uint32_t ent[32];
-  const size_t L = round_levels(n, ent, 32);
-  const uint32_t blocks1 = std::max(1u, (ent[0] + level_span(1) - 1) / level_span(1));
+  // the accumulators hold reduce_blocks(n) + 1 slots; the grid size names the giant's slot
+  const XAcc xa{rb.xacc, rb.xcnt, rb.off, rb.cursor};
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_bv_reduce_first<uint32_t>, dim3(blocks1), dim3(kRedBlock), 0, s, st, members, rb.root, slab,
-                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats,
-                       rb.det_cpy);
+    hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(reduce_blocks(n)), dim3(kRedBlock), 0, s, st, members, rb.root,
+                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy);
   else
-    hipLaunchKernelGGL(k_bv_reduce_first<uint64_t>, dim3(blocks1), dim3(kRedBlock), 0, s, st, members, rb.root, slab,
-                       g, round, rb.part[0], rb.proot[0], rb.samp, d.flags, prev, rb.rstats,
-                       rb.det_cpy);
-  for (size_t l = 1; l < L; ++l) {  // level l + 1
-    const uint32_t blocks = std::max(1u, (ent[l] + level_span((uint32_t)l + 1) - 1) / level_span((uint32_t)l + 1));
-    const bool tail = l + 1 >= L || ent[l + 1] <= kTailEntries;
-    const uint32_t level = (uint32_t)l + 1;
-    if (gbytes == 4)
-      hipLaunchKernelGGL(k_bv_reduce_up<uint32_t>, dim3(blocks), dim3(kRedBlock), 0, s, level, tail ? 1u : 0u, st, g,
-                         round, rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
-    else
-      hipLaunchKernelGGL(k_bv_reduce_up<uint64_t>, dim3(blocks), dim3(kRedBlock), 0, s, level, tail ? 1u : 0u, st, g,
-                         round, rb.part[0], rb.part[1], rb.proot[0], rb.proot[1], rb.samp, d.flags, prev);
-    if (tail) break;
-  }
+    hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(reduce_blocks(n)), dim3(kRedBlock), 0, s, st, members, rb.root,
+                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
                      act_list, prev, cur);
 }

@@ -84,11 +84,11 @@ struct RoundBufs {
   uint32_t* root;        // n      root of every walked vertex this round
   uint32_t* cnt;         // n      members per root (zero between rounds)
   uint32_t* off;         // n      segment start per root
-  uint32_t* cursor;      // n      scatter cursor per root
+  uint32_t* cursor;      // n      scatter cursor per root (the segment end after the scatter)
   uint32_t* members[2];  // n each vertices grouped by root, by round parity (also the next active list)
   uint32_t* totals;      // 8      round state words (entry counts, giant roots, walk mode)
-  uint64_t* part[2];     // partial-XOR entries of the multi-level reduce, 3*ceil(n/64)+2 entries of 128 u64
-  uint32_t* proot[2];    // root of each partial entry
+  uint64_t* xacc;        // (reduce_blocks(n) + 1) * 128: cross-workgroup supernode sums, zero between rounds
+  uint32_t* xcnt;        // reduce_blocks(n) + 1 arrival counters, zero between rounds
   uint64_t* samp;        // n      sample record per supernode root
   uint32_t* rstats;      // 2 * 64 per round: vertices regrouped, giant root (diagnostics)
   unsigned char* det_cpy;  // n records: every vertex's deterministic bucket, copied in round 0
@@ -96,8 +96,8 @@ struct RoundBufs {
 // Writes rb.samp: a sample record per vertex.
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
                           hipStream_t s);
-// Number of reduce levels for n vertices (entries_per_level gets a bound on each level's entries).
-size_t round_levels(uint32_t n, uint32_t* entries_per_level, size_t max_levels);
+// Workgroups of a round's reduce over n vertices (1024 member entries each).
+uint32_t reduce_blocks(uint32_t n);
 // Queue round `round` (>= 1); every kernel of it is a no-op when d.mod[round - 1] == 0.
 // giant_ok: the vertex sketches are known to XOR to zero, so a supernode holding more than half of
 // the vertices may be summed from its complement.

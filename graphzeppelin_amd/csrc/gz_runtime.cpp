@@ -172,9 +172,9 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ + part +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
   // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
-  // forest edge (8 each), partial entries (2 buffers x 3 x 1 KiB per 64 vertices), spanning-forest
-  // hash set (16)
-  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 96 + 16 + 8 + gbytes) + 8192;  // + deterministic copy
+  // forest edge (8 each), spanning-forest hash set (16), deterministic-bucket copy; the reduce's
+  // cross-workgroup accumulators (1 KiB + 4 B per 1024 vertices, and one more slot)
+  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 16 + 8 + gbytes) + ((uint64_t)n / 1024 + 2) * 1028 + 8192;
   return slab + ingest + rounds;
 }
 
@@ -224,21 +224,20 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   launch_dsu_reset(dsu, st);
 
   // Borůvka round buffers
-  const uint32_t waves1 = (n + 63) / 64;
   r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
   r_members[0].alloc(n); r_members[1].alloc(n);
   r_totals.alloc(8);
   r_stats.alloc(128);
   r_det.alloc((size_t)n * (8 + gb));
   launch_fill_u32(r_stats.p, 0, 128, st);
-  for (int i = 0; i < 2; ++i) {
-    // level 1 writes head and tail partials per wave, plus one total per wave when a supernode
-    // is summed from its complement (which may hold as few as n / 32 vertices): 3 per wave
-    r_part[i].alloc((size_t)(3 * waves1 + 2) * 128);
-    r_proot[i].alloc(3 * waves1 + 2);
-  }
+  // cross-workgroup accumulators of the reduce: one slot per workgroup and one for the giant
+  const size_t xslots = (size_t)reduce_blocks(n) + 1;
+  r_xacc.alloc(xslots * 128);
+  r_xcnt.alloc(xslots);
+  launch_fill_u64(r_xacc.p, 0, xslots * 128, st);
+  launch_fill_u32(r_xcnt.p, 0, xslots, st);
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
-                 {r_part[0].p, r_part[1].p}, {r_proot[0].p, r_proot[1].p}, r_samp.p, r_stats.p, r_det.p};
+                 r_xacc.p, r_xcnt.p, r_samp.p, r_stats.p, r_det.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest
