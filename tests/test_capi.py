@@ -51,7 +51,7 @@ def test_footprint_grows_with_n():
     a = gz.footprint_bytes(1 << 12)
     b = gz.footprint_bytes(1 << 18)
     assert 0 < a < b
-    # the sketch slab dominates: 2^18 vertices x 901 buckets x 12 bytes (rows padded to 32 buckets)
+    # the sketch slab dominates: 2^18 vertices x 901 buckets x 12 bytes (plus each column's padding to a cache line)
     assert b > (1 << 18) * 901 * 12
 
 
