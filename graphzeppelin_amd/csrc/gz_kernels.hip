@@ -1029,24 +1029,27 @@ struct XAcc {
 // Reduce entries [p0, p0 + len) (len <= 64) with one wave.  prev_root / next_root: roots of the
 // entries just before and after the range (kNoRoot when none), which decide whether the first and
 // last segments are open.  FIRST: entries are members (rows of the slab); otherwise partial
-// entries (in_part / in_root in LDS).  *all_a / *all_g get the XOR of every entry read.  FIRST
+// entries (in_part / in_root in LDS).  FIRST with ROOTPOS: the entries' roots by position in
+// root_pos (members and root_pos may be in LDS) instead of by vertex in vroot.  *all_a / *all_g get the XOR
+// of every entry read.  FIRST
 // with s_bnd: the wave holding the workgroup's first entry [blk_b0] loads that entry's segment
 // bounds into s_bnd[0..1], the one holding its last entry the end of that entry's segment into
 // s_bnd[2], in flight with the values.
-template <class G, bool FIRST>
+template <class G, bool FIRST, bool ROOTPOS = false>
 __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_root, uint32_t next_root,
                                     const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
                                     const unsigned char* __restrict__ slab, const Geo& g, uint32_t round,
                                     const uint64_t* in_part, const uint32_t* in_root, Slot head, Slot tail,
                                     uint64_t* __restrict__ samp, uint32_t* __restrict__ flags, uint64_t* all_a_out,
                                     uint64_t* all_g_out, const unsigned char* __restrict__ det_cpy,
-                                    const XAcc* xa, uint32_t blk_b0, uint32_t blk_b1, uint32_t* s_bnd) {
+                                    const XAcc* xa, uint32_t blk_b0, uint32_t blk_b1, uint32_t* s_bnd,
+                                    const uint32_t* root_pos = nullptr) {
   const uint32_t lane = lane_id();
   uint32_t m = 0, rt = kNoRoot;
   if (lane < len) {
     if (FIRST) {
       m = members[p0 + lane];
-      rt = vroot[m];
+      rt = ROOTPOS ? root_pos[p0 + lane] : vroot[m];
     } else {
       rt = in_root[p0 + lane];
     }
@@ -1280,6 +1283,207 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
     st_rlx(sv + lane, (uint64_t)0);
     st_rlx(sv + kWave + lane, (uint64_t)0);
     if (lane == 0) st_rlx(xa.cnt + slot, 0u);
+  }
+}
+
+// ---- the last rounds in one workgroup ----------------------------------------------------------
+// Once a giant supernode exists, a round walks only the vertices outside it, and after a few rounds
+// they are few.  k_bv_finish then runs every remaining round inside one workgroup -- the same
+// regrouping, reduce, sample and merge as k_bv_find_count .. k_apply_samples, with barriers between
+// the steps instead of kernel boundaries -- so a round costs a few round trips instead of five
+// dependent launches.  It takes over at round r0 only when round r0 - 1 changed something, a giant
+// exists (or was recorded as the next round's candidate) and the walk holds at most kFinishMax
+// vertices; otherwise it returns without touching any state and the rounds continue as launches.
+// It then sets mod[r] for every modified round, as the launched rounds do, and records the last
+// round it ran in mod[nmod - 1] (a word no round uses), so one copy of mod tells the host both.
+//
+// The workgroup reduces its grouped list in chunks of 64 entries (one wave each, reduce_range).  A
+// supernode continuing past a chunk is accumulated in LDS under the chunk where it starts (only one
+// supernode per chunk can continue past its end) and sampled after a barrier.  DSU and sample
+// loads that an earlier round of this kernel may have changed are device-coherent loads.
+constexpr uint32_t kFinishMax = 2048;
+constexpr uint32_t kFinishTable = 4096;  // LDS hash table of roots, >= 2 * kFinishMax, power of two
+constexpr uint32_t kFinishChunks = kFinishMax / kWave;
+constexpr uint32_t kFinishPer = kFinishMax / kRedBlock;  // walked vertices per thread
+
+__device__ inline uint32_t dsu_find_coherent(uint32_t* parent, uint32_t u) {
+  uint32_t p = ld_acq(parent + u);
+  while (true) {
+    const uint32_t gp = ld_acq(parent + p);
+    if (gp == p) return p;
+    u = p;
+    p = gp;
+  }
+}
+
+__device__ inline uint32_t finish_slot(uint32_t* keys, uint32_t key) {
+  uint32_t h = (key * 2654435761u) & (kFinishTable - 1);
+  while (true) {
+    const uint32_t prev = atomicCAS(keys + h, kEmptyKey, key);
+    if (prev == kEmptyKey || prev == key) return h;
+    h = (h + 1) & (kFinishTable - 1);
+  }
+}
+
+template <class G>
+__global__ void __launch_bounds__(kRedBlock)
+k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ act0,
+            const unsigned char* __restrict__ slab, Geo g, uint32_t r0, uint32_t last_round,
+            uint64_t* __restrict__ samp, uint32_t* __restrict__ rstats, const unsigned char* __restrict__ det_cpy) {
+  __shared__ uint32_t s_key[kFinishTable], s_cnt[kFinishTable];
+  __shared__ uint32_t s_mem[kFinishMax], s_mroot[kFinishMax], s_seg[kFinishMax];
+  __shared__ uint64_t s_acc[kFinishChunks * 2 * kWave];
+  __shared__ uint32_t s_accroot[kFinishChunks];
+  __shared__ uint64_t s_slot[kRedWaves * 2 * 2 * kWave];  // per wave: head and tail of its chunk
+  __shared__ uint32_t s_slotroot[kRedWaves * 2];
+  __shared__ uint64_t s_tot[2 * kWave];
+  __shared__ uint32_t s_wsum[kRedWaves];
+  __shared__ uint32_t s_take, s_gfirst, s_walk, s_gcur, s_e;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+  if (tid == 0) {
+    const uint32_t live = ld_acq(d.mod + (r0 - 1));
+    uint32_t gfirst = st[kStGiantFirst];
+    const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
+    if (gfirst == kNoRoot && best) gfirst = (uint32_t)best;  // as k_bv_find_count takes the candidate
+    const uint32_t walk = st[kStSeg + ((r0 - 1) & 1)];
+    s_take = (live && gfirst != kNoRoot && walk <= kFinishMax) ? 1u : 0u;
+    s_gfirst = gfirst;
+    s_walk = walk;
+  }
+  __syncthreads();
+  if (!s_take) return;
+  uint32_t walk = s_walk;
+  uint32_t v[kFinishPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kFinishPer; ++k) v[k] = tid + k * kRedBlock < walk ? act0[tid + k * kRedBlock] : 0u;
+  for (uint32_t c = tid; c < kFinishChunks * 2 * kWave; c += kRedBlock) s_acc[c] = 0;
+  for (uint32_t c = tid; c < kFinishChunks; c += kRedBlock) s_accroot[c] = kNoRoot;
+
+  for (uint32_t r = r0;; ++r) {
+    for (uint32_t k = tid; k < kFinishTable; k += kRedBlock) {
+      s_key[k] = kEmptyKey;
+      s_cnt[k] = 0;
+    }
+    if (tid < 2 * kWave) s_tot[tid] = 0;
+    if (tid == 0) s_gcur = dsu_find_coherent(d.parent, s_gfirst);  // the giant only grows
+    __syncthreads();
+    const uint32_t gcur = s_gcur;
+    // roots of the walked vertices, full path compression, members per root (k_bv_find_count)
+    uint32_t rr[kFinishPer], slot[kFinishPer], pos[kFinishPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kFinishPer; ++k) rr[k] = tid + k * kRedBlock < walk ? dsu_find_coherent(d.parent, v[k]) : kNoRoot;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kFinishPer; ++k) {
+      if (rr[k] == kNoRoot) continue;
+      st_rlx(d.parent + v[k], rr[k]);
+      if (rr[k] != gcur) {  // members of the giant are not regrouped
+        slot[k] = finish_slot(s_key, rr[k]);
+        pos[k] = atomicAdd(s_cnt + slot[k], 1u);
+      }
+    }
+    __syncthreads();
+    // segment starts: exclusive scan of the counts over the table (4 slots per thread)
+    {
+      constexpr uint32_t per = kFinishTable / kRedBlock;
+      uint32_t c[per], sum = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < per; ++k) {
+        c[k] = s_cnt[tid * per + k];
+        sum += c[k];
+      }
+      const uint32_t incl = (uint32_t)wave_incl_scan_u64(sum);
+      if (lane == kWave - 1) s_wsum[wid] = incl;
+      __syncthreads();
+      uint32_t pre = 0, tot = 0;
+      for (uint32_t w = 0; w < kRedWaves; ++w) {
+        if (w < wid) pre += s_wsum[w];
+        tot += s_wsum[w];
+      }
+      uint32_t o = pre + incl - sum;
+#pragma unroll
+      for (uint32_t k = 0; k < per; ++k) {
+        s_cnt[tid * per + k] = o;  // now the segment start
+        o += c[k];
+      }
+      if (tid == 0) s_e = tot;
+    }
+    __syncthreads();
+    const uint32_t e = s_e;
+#pragma unroll
+    for (uint32_t k = 0; k < kFinishPer; ++k) {
+      if (rr[k] == kNoRoot || rr[k] == gcur) continue;
+      const uint32_t p = s_cnt[slot[k]] + pos[k];
+      s_mem[p] = v[k];
+      s_mroot[p] = rr[k];
+      s_seg[p] = s_cnt[slot[k]];
+    }
+    if (tid == 0 && rstats && r < kMaxStatRounds) {
+      rstats[2 * r] = e;
+      rstats[2 * r + 1] = gcur;
+    }
+    __syncthreads();
+    // reduce + sample (k_bv_reduce): one wave per chunk of 64 entries
+    const uint32_t nchunks = (e + kWave - 1) / kWave;
+    Slot hs{s_slot + (size_t)(2 * wid) * 2 * kWave, s_slotroot + 2 * wid};
+    Slot ts{s_slot + (size_t)(2 * wid + 1) * 2 * kWave, s_slotroot + 2 * wid + 1};
+    for (uint32_t c = wid; c < nchunks; c += kRedWaves) {
+      const uint32_t p0 = c * kWave;
+      const uint32_t len = e - p0 < kWave ? e - p0 : kWave;
+      const uint32_t prev = p0 > 0 ? s_mroot[p0 - 1] : kNoRoot;
+      const uint32_t next = p0 + len < e ? s_mroot[p0 + len] : kNoRoot;
+      uint64_t wa, wg;
+      reduce_range<G, true, true>(p0, len, prev, next, s_mem, nullptr, slab, g, r, nullptr, nullptr, hs, ts, samp, d.flags,
+                            &wa, &wg, det_cpy, nullptr, 0, 0, nullptr, s_mroot);
+      lds_xor(s_tot + lane, wa);
+      lds_xor(s_tot + kWave + lane, wg);
+      const uint32_t hr = *hs.root, tr = *ts.root;
+      if (hr != kNoRoot) {  // started in an earlier chunk
+        uint64_t* acc = s_acc + (size_t)(s_seg[p0] / kWave) * 2 * kWave;
+        lds_xor(acc + lane, hs.vals[lane]);
+        lds_xor(acc + kWave + lane, hs.vals[kWave + lane]);
+      }
+      if (tr != kNoRoot && tr != hr) {  // starts here and continues
+        uint64_t* acc = s_acc + (size_t)c * 2 * kWave;
+        lds_xor(acc + lane, ts.vals[lane]);
+        lds_xor(acc + kWave + lane, ts.vals[kWave + lane]);
+        if (lane == 0) s_accroot[c] = tr;
+      }
+    }
+    __syncthreads();
+    for (uint32_t c = wid; c < nchunks; c += kRedWaves) {
+      const uint32_t R = s_accroot[c];
+      if (R == kNoRoot) continue;
+      uint64_t* acc = s_acc + (size_t)c * 2 * kWave;
+      uint64_t idx;
+      const int code = wave_sample<G>(acc[lane], (G)acc[kWave + lane], g.D, g.chk_bf, &idx);
+      if (lane == 0) samp[R] = sample_record(code, idx, g.n, d.flags);
+      acc[lane] = 0;  // zero for the next round
+      acc[kWave + lane] = 0;
+      if (lane == 0) s_accroot[c] = kNoRoot;
+    }
+    if (wid == kRedWaves - 1) {  // the giant, from the complement
+      uint64_t idx;
+      const int code = wave_sample<G>(s_tot[lane], (G)s_tot[kWave + lane], g.D, g.chk_bf, &idx);
+      if (lane == 0) samp[gcur] = e == 0 ? kSampZero : sample_record(code, idx, g.n, d.flags);
+    }
+    __syncthreads();
+    // merges (k_apply_samples): every supernode root of the walk, and the giant
+    bool modified = false;
+    for (uint32_t k = tid; k < kFinishTable; k += kRedBlock)
+      if (s_key[k] != kEmptyKey && apply_sample(d, ld_acq(samp + s_key[k]))) modified = true;
+    if (tid == 0 && apply_sample(d, ld_acq(samp + gcur))) modified = true;
+    const bool any = __syncthreads_or(modified);
+    if (tid == 0 && any) st_rlx(d.mod + r, 1u);
+    if (!any || r >= last_round) {
+      if (tid == 0) st_rlx(d.mod + (d.nmod - 1), r);
+      return;
+    }
+    // the next round walks this round's members
+    walk = e;
+#pragma unroll
+    for (uint32_t k = 0; k < kFinishPer; ++k) v[k] = tid + k * kRedBlock < walk ? s_mem[tid + k * kRedBlock] : 0u;
+    __syncthreads();
   }
 }
 
@@ -1578,6 +1782,19 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
                      act_list, prev, cur);
 }
+
+void launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned char* slab, const Geo& g,
+                   const DsuDev& d, const RoundBufs& rb, hipStream_t s) {
+  if (r0 < 2 || r0 > last_round) return;  // round r0 walks round r0 - 1's member list
+  const uint32_t* act0 = rb.members[(r0 - 1) & 1];
+  if (gbytes == 4)
+    hipLaunchKernelGGL(k_bv_finish<uint32_t>, dim3(1), dim3(kRedBlock), 0, s, d, rb.totals, act0, slab, g, r0,
+                       last_round, rb.samp, rb.rstats, rb.det_cpy);
+  else
+    hipLaunchKernelGGL(k_bv_finish<uint64_t>, dim3(1), dim3(kRedBlock), 0, s, d, rb.totals, act0, slab, g, r0,
+                       last_round, rb.samp, rb.rstats, rb.det_cpy);
+}
+uint32_t finish_max() { return kFinishMax; }
 
 void launch_round_start(const RoundBufs& rb, hipStream_t s) {
   launch_fill_u32(rb.totals, 0, kStWords, s);

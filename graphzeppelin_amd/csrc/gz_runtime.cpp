@@ -272,6 +272,8 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   eager_cap = size_t(1) << 20;
   hflags.alloc(std::max<size_t>(kNumFlags, geo.samples + 2));
   hsmall.alloc(16);
+  hround.alloc(128);
+  GZ_HIP(hipEventCreateWithFlags(&hround_ev, hipEventDisableTiming));
   GZ_HIP(hipEventCreateWithFlags(&eager_ev, hipEventDisableTiming));
   GZ_HIP(hipStreamSynchronize(st));
 }
@@ -286,6 +288,7 @@ GzAlg::~GzAlg() {
   for (auto& s : stages)
     if (s.ev) (void)hipEventDestroy(s.ev);
   if (eager_ev) (void)hipEventDestroy(eager_ev);
+  if (hround_ev) (void)hipEventDestroy(hround_ev);
   for (auto& kv : query_graphs) (void)hipGraphExecDestroy(kv.second);
   if (st) (void)hipStreamDestroy(st);
 }
@@ -315,7 +318,7 @@ void GzAlg::pre_insert(uint32_t src, uint32_t dst) {
 void GzAlg::submit_eager_locked() {
   if (!eager_used) return;
   GZ_HIP(hipMemcpyAsync(eager_dev.p, eager_host.p, eager_used * 8, hipMemcpyHostToDevice, st));
-  launch_eager(eager_dev.p, eager_used, eager_merged.p, dsu, st);
+  eager_batch(eager_dev.p, eager_used, eager_merged.p);
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipEventRecord(eager_ev, st));
   GZ_HIP(hipEventSynchronize(eager_ev));  // the host buffer is reused right away
@@ -518,7 +521,7 @@ void GzAlg::ingest_device(const uint32_t* pairs_dev, size_t count) {
     const size_t take = std::min(count, cap_pairs - pending_n);
     uint32_t* dst = pending.p + 2 * pending_n;
     GZ_HIP(hipMemcpyAsync(dst, pairs_dev, take * 8, hipMemcpyDeviceToDevice, st));
-    if (cfg.eager_dsu && dsu_valid) launch_eager(dst, take, merged.p, dsu, st);
+    if (cfg.eager_dsu && dsu_valid) eager_batch(dst, take, merged.p);
     pending_n += take;
     pairs_dev += 2 * take;
     count -= take;
@@ -536,7 +539,7 @@ void GzAlg::ingest_host(const uint32_t* pairs, size_t count) {
     uint32_t* dst = pending.p + 2 * pending_n;
     GZ_HIP(hipMemcpyAsync(dst, pairs, take * 8, hipMemcpyHostToDevice, st));
     GZ_HIP(hipStreamSynchronize(st));
-    if (cfg.eager_dsu && dsu_valid) launch_eager(dst, take, merged.p, dsu, st);
+    if (cfg.eager_dsu && dsu_valid) eager_batch(dst, take, merged.p);
     pending_n += take;
     pairs += 2 * take;
     count -= take;
@@ -616,14 +619,16 @@ bool GzAlg::sketches_sum_to_zero() {
   return zero_sum == 1;
 }
 
-// DSU reset, round 0 and rounds 1..last, as one stream sequence (captured into a graph or launched
-// directly).  Fills are kernels, so the sequence holds no memset nodes.
+// DSU reset, round 0, rounds 1..last and (with a giant allowed) the single-workgroup finish from
+// round last + 1, as one stream sequence (captured into a graph or launched directly).  Fills are
+// kernels, so the sequence holds no memset nodes.
 void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
   launch_dsu_reset(dsu, st);
   launch_fill_u32(r_cnt.p, 0, n, st);
   launch_round_start(rb, st);
   launch_sample_round0(gb, slab.p, geo, dsu, rb, st);
   for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
+  if (giant_ok) launch_finish(gb, last + 1, geo.samples - 1, slab.p, geo, dsu, rb, st);
 }
 
 hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
@@ -649,15 +654,31 @@ hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
 
 // boruvka_emulation, src/cc_sketch_alg.cpp:464-513.  Every kernel of round r is a no-op once
 // round r-1 changed nothing, so rounds are queued ahead and the host reads the per-round flags
-// once per group: first the graph prefix (round 0 and rounds 1..K), then groups of 2.
+// once per group: first the graph prefix (round 0, rounds 1..K and a finish attempt), then groups
+// of 2 rounds, each followed by a finish attempt.  A finish that takes over runs every remaining
+// round, so at most one does per query.
 void GzAlg::boruvka() {
   update_locked = true;
   try {
     const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
     const bool giant_ok = sketches_sum_to_zero();
     const uint32_t last_round = geo.samples - 1;  // round index `samples` would be out of samples
-    const uint32_t want = last_rounds ? (uint32_t)last_rounds + 1 : kFirstGraphRounds;
+    if (hround_pending) {
+      // The launched rounds of this query: up to the first round of the previous query after which
+      // a giant existed and at most finish_max() vertices remained outside it (diagnostics: vertices
+      // regrouped and the giant's root per round).
+      GZ_HIP(hipEventSynchronize(hround_ev));
+      hround_pending = false;
+      next_prefix = (uint32_t)last_rounds + 1;
+      for (uint32_t r = 1; r < (uint32_t)last_rounds && r < 64; ++r)
+        if (hround.p[2 * r + 1] != 0xFFFFFFFFu && hround.p[2 * r] <= finish_max()) {
+          next_prefix = r;
+          break;
+        }
+    }
+    const uint32_t want = next_prefix ? next_prefix : kFirstGraphRounds;
     const uint32_t first = std::max(1u, std::min(want, last_round));
+    sf_pending = false;  // the DSU reset clears the set
     if (prof) prof_begin(1, 0, false, 0);
     if (use_graphs) GZ_HIP(hipGraphLaunch(get_query_graph(giant_ok, first), st));
     else enqueue_query_prefix(giant_ok, first);
@@ -665,9 +686,10 @@ void GzAlg::boruvka() {
     uint32_t next = first + 1;
     int64_t rounds = -1;
     while (true) {
-      const uint32_t upto = next - 1;
-      GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)(upto + 1) * 4, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)dsu.nmod * 4, hipMemcpyDeviceToHost, st));
       GZ_HIP(hipStreamSynchronize(st));
+      // rounds launched so far, or as far as a finish that took over went
+      const uint32_t upto = std::max(next - 1, hflags.p[dsu.nmod - 1]);
       for (uint32_t r = 0; r <= upto; ++r)
         if (!hflags.p[r]) {
           rounds = r;
@@ -678,16 +700,20 @@ void GzAlg::boruvka() {
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
       const uint32_t hi = std::min(next + 1, last_round);
       for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
+      if (giant_ok) launch_finish(gb, hi + 1, last_round, slab.p, geo, dsu, rb, st);
       GZ_HIP(hipGetLastError());
       next = hi + 1;
     }
-    if (cfg.eager_dsu) launch_sf_build(dsu, st);  // the eager DSU's deletion check needs the set
     if (prof) {
       prof_end();
       timed.back().host_bytes = round_bytes * (double)(rounds + 1);
       timed.back().items = (uint64_t)rounds + 1;
     }
     last_rounds = (size_t)rounds;
+    GZ_HIP(hipMemcpyAsync(hround.p, r_stats.p, 128 * 4, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipEventRecord(hround_ev, st));
+    hround_pending = true;
+    sf_pending = cfg.eager_dsu;
     dsu_valid = true;
     if (prof) prof_collect();
   } catch (...) {
@@ -696,6 +722,14 @@ void GzAlg::boruvka() {
     throw;
   }
   update_locked = false;
+}
+
+void GzAlg::eager_batch(const uint32_t* pairs, size_t count, uint8_t* merged) {
+  if (sf_pending) {
+    launch_sf_build(dsu, st);
+    sf_pending = false;
+  }
+  launch_eager(pairs, count, merged, dsu, st);
 }
 
 // connected_components, src/cc_sketch_alg.cpp:515-557 + ConnectedComponents, return_types.cpp:5-19

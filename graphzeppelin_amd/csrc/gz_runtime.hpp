@@ -249,16 +249,25 @@ class GzAlg {
   void boruvka();
   uint32_t read_flag(int slot);
 
-  // Query prefix (DSU reset, round 0 and rounds 1..K) replayed from a HIP graph: every parameter is
-  // fixed per instance (round state lives on the device), so one graph launch replaces ~9 kernel
-  // launches per round.  K follows the previous query's round count plus one (queries on one
-  // stream need similar counts), so few no-op rounds are dispatched; graphs are built on first use
-  // and kept per (K, giant_ok).
+  // Query prefix (DSU reset, round 0, rounds 1..K and a single-workgroup finish of the remaining
+  // rounds) replayed from a HIP graph: every parameter is fixed per instance (round state lives on
+  // the device), so one graph launch replaces ~5 kernel launches per round.  K is the last round
+  // the previous query needed launched rounds for: the first round after which a giant existed and
+  // at most finish_max() vertices remained outside it (queries on one stream need similar counts),
+  // else its round count plus one.  Graphs are built on first use and kept per (K, giant_ok).
  public:
   bool use_graphs = true;
   static constexpr uint32_t kFirstGraphRounds = 12;
  private:
   std::map<uint32_t, hipGraphExec_t> query_graphs;
+  PinBuf<uint32_t> hround;  // per-round diagnostics of the last query (copied when it ends)
+  hipEvent_t hround_ev = nullptr;  // that copy has landed
+  bool hround_pending = false;
+  uint32_t next_prefix = 0;  // K of the next query's prefix (0: none chosen yet)
+  // The spanning-forest membership set that the eager DSU's deletion check reads is built from the
+  // forest edges after a query, but only before the first eager batch that needs it.
+  bool sf_pending = false;
+  void eager_batch(const uint32_t* pairs, size_t count, uint8_t* merged);
   void enqueue_query_prefix(bool giant_ok, uint32_t last);
   hipGraphExec_t get_query_graph(bool giant_ok, uint32_t last);
 };
