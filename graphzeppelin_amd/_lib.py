@@ -159,6 +159,8 @@ _SIGS = {
     "gz_alg_digest": (ctypes.c_int, [_vp, _u64p]),
     "gz_alg_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_alg_set_query_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_alg_set_round_finish": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_alg_last_query_finish_round": (ctypes.c_uint32, [_vp]),
     "gz_alg_last_query_round_sizes": (_sz, [_vp, _vp, _vp, _sz]),
     "gz_alg_get_profile": (ctypes.c_int, [_vp, ctypes.POINTER(AlgProfile)]),
     # include/gz_serving.h
@@ -541,6 +543,15 @@ class CCSketchAlg:
     def set_query_graphs(self, enable: bool = True):
         """Replay each query's first Borůvka rounds from a HIP graph (default) or launch them one by one."""
         _check(lib().gz_alg_set_query_graphs(self._h, 1 if enable else 0))
+
+    def set_round_finish(self, enable: bool = True):
+        """Run the last Borůvka rounds in one single-workgroup launch (default) or as separate launches."""
+        _check(lib().gz_alg_set_round_finish(self._h, 1 if enable else 0))
+
+    @property
+    def last_query_finish_round(self) -> int:
+        """Round from which the last query ran its remaining rounds in one workgroup (0: it did not)."""
+        return int(lib().gz_alg_last_query_finish_round(self._h))
 
     def get_profile(self) -> dict:
         p = AlgProfile()

@@ -1070,10 +1070,12 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
   const bool loads = lane <= g.D;
   bool wrote_head = false, wrote_tail = false;
 
+  // Every lane stores the root (the same value), so every lane of this wave may read it back
+  // without a barrier.
   auto put = [&](Slot sl, uint32_t R, uint64_t a, uint64_t gm) {
     sl.vals[lane] = a;
     sl.vals[kWave + lane] = gm;
-    if (lane == 0) *sl.root = R;
+    *sl.root = R;
   };
   auto emit = [&](uint32_t R, uint64_t acc_a, uint64_t acc_g, uint32_t seg_start, uint32_t seg_end) {
     const bool started_before = seg_start == p0 && head_open;
@@ -1467,6 +1469,9 @@ k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ ac
       const int code = wave_sample<G>(s_tot[lane], (G)s_tot[kWave + lane], g.D, g.chk_bf, &idx);
       if (lane == 0) samp[gcur] = e == 0 ? kSampZero : sample_record(code, idx, g.n, d.flags);
     }
+    // The samples were written with plain stores; the merges read them with device-coherent loads,
+    // which a workgroup barrier alone does not order after those stores.
+    __threadfence();
     __syncthreads();
     // merges (k_apply_samples): every supernode root of the walk, and the giant
     bool modified = false;

@@ -628,11 +628,12 @@ void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
   launch_round_start(rb, st);
   launch_sample_round0(gb, slab.p, geo, dsu, rb, st);
   for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
-  if (giant_ok) launch_finish(gb, last + 1, geo.samples - 1, slab.p, geo, dsu, rb, st);
+  if (giant_ok && use_finish) launch_finish(gb, last + 1, geo.samples - 1, slab.p, geo, dsu, rb, st);
 }
 
 hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
-  hipGraphExec_t& slot = query_graphs[last * 2 + (giant_ok ? 1 : 0)];
+  const uint32_t key = (last * 2 + (giant_ok ? 1 : 0)) * 2 + (use_finish ? 1 : 0);
+  hipGraphExec_t& slot = query_graphs[key];
   if (slot) return slot;
   GZ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   enqueue_query_prefix(giant_ok, last);
@@ -640,13 +641,13 @@ hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
   const hipError_t cap = hipStreamEndCapture(st, &graph);
   if (cap != hipSuccess || !graph) {
     if (graph) (void)hipGraphDestroy(graph);
-    query_graphs.erase(last * 2 + (giant_ok ? 1 : 0));
+    query_graphs.erase(key);
     throw GzError(GZ_ERR_DEVICE, std::string("query graph capture: ") + hipGetErrorString(cap));
   }
   const hipError_t inst = hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (inst != hipSuccess) {
-    query_graphs.erase(last * 2 + (giant_ok ? 1 : 0));
+    query_graphs.erase(key);
     throw GzError(GZ_ERR_DEVICE, std::string("query graph instantiate: ") + hipGetErrorString(inst));
   }
   return slot;
@@ -684,6 +685,7 @@ void GzAlg::boruvka() {
     else enqueue_query_prefix(giant_ok, first);
     GZ_HIP(hipGetLastError());
     uint32_t next = first + 1;
+    uint32_t fin_r0 = first + 1;  // first round of the last finish attempt queued
     int64_t rounds = -1;
     while (true) {
       GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)dsu.nmod * 4, hipMemcpyDeviceToHost, st));
@@ -700,8 +702,9 @@ void GzAlg::boruvka() {
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
       const uint32_t hi = std::min(next + 1, last_round);
       for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
-      if (giant_ok) launch_finish(gb, hi + 1, last_round, slab.p, geo, dsu, rb, st);
+      if (giant_ok && use_finish) launch_finish(gb, hi + 1, last_round, slab.p, geo, dsu, rb, st);
       GZ_HIP(hipGetLastError());
+      fin_r0 = hi + 1;
       next = hi + 1;
     }
     if (prof) {
@@ -710,6 +713,7 @@ void GzAlg::boruvka() {
       timed.back().items = (uint64_t)rounds + 1;
     }
     last_rounds = (size_t)rounds;
+    last_finish_start = hflags.p[dsu.nmod - 1] ? fin_r0 : 0;
     GZ_HIP(hipMemcpyAsync(hround.p, r_stats.p, 128 * 4, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipEventRecord(hround_ev, st));
     hround_pending = true;
@@ -1207,6 +1211,13 @@ int gz_alg_set_query_graphs(gz_alg* a, int enable) {
     a->a->use_graphs = enable != 0;
   });
 }
+int gz_alg_set_round_finish(gz_alg* a, int enable) {
+  return guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    a->a->use_finish = enable != 0;
+  });
+}
+uint32_t gz_alg_last_query_finish_round(const gz_alg* a) { return a ? a->a->last_finish_start : 0; }
 int gz_alg_get_profile(gz_alg* a, gz_alg_profile* out) {
   return guarded([&] {
     GZ_REQUIRE(a && out, "null argument");

@@ -257,6 +257,8 @@ class GzAlg {
   // else its round count plus one.  Graphs are built on first use and kept per (K, giant_ok).
  public:
   bool use_graphs = true;
+  bool use_finish = true;           // the single-workgroup finish of the last rounds
+  uint32_t last_finish_start = 0;   // round at which it took over in the last query (0: none)
   static constexpr uint32_t kFirstGraphRounds = 12;
  private:
   std::map<uint32_t, hipGraphExec_t> query_graphs;

@@ -175,6 +175,14 @@ int gz_alg_set_query_graphs(gz_alg* a, int enable);
  * the last query that ran Borůvka: the vertices regrouped (those outside a giant supernode) and
  * the giant's root (0xFFFFFFFF when none).  Returns the number of rounds written (<= cap). */
 size_t gz_alg_last_query_round_sizes(gz_alg* a, uint32_t* entries, uint32_t* giant_root, size_t cap);
+/* No counterpart in the reference (a device execution knob).  enable != 0 (the default): once a
+ * giant supernode exists and at most 2048 vertices remain outside it, one single-workgroup launch
+ * runs every remaining Borůvka round; 0: every round runs as its own launches.  Results are
+ * identical. */
+int gz_alg_set_round_finish(gz_alg* a, int enable);
+/* No counterpart in the reference (diagnostics).  The Borůvka round from which the last query ran
+ * its remaining rounds in the single-workgroup finish, or 0 when it did not. */
+uint32_t gz_alg_last_query_finish_round(const gz_alg* a);
 /* Device footprint in bytes of an instance with these parameters (sketch slab + work buffers). */
 uint64_t gz_alg_footprint_bytes(uint32_t num_vertices, const gz_alg_config* cfg);
 

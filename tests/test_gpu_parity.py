@@ -332,6 +332,34 @@ def test_query_graph_replays_equal_direct_launches():
         assert g.point_query(0, 1) == d.point_query(0, 1)
 
 
+def test_round_finish_equals_launched_rounds():
+    # Late rounds, once few vertices remain outside the giant, run inside one workgroup; queries
+    # with that finish and with every round launched separately must agree round for round, and
+    # with the exact components.  The first query sizes the next queries' launched rounds, so the
+    # finish takes over from the second query on.
+    pairs, _ = streams.dynamic_erdos_stream(1500, 0.0012, 0.3, 0.2, 3, 21)
+    f = gz.CCSketchAlg(1500, 77, gz.CCAlgConfiguration(eager_dsu=False))
+    l = gz.CCSketchAlg(1500, 77, gz.CCAlgConfiguration(eager_dsu=False))
+    l.set_round_finish(False)
+    took_over = 0
+    step = pairs.shape[0] // 8 + 1
+    for i in range(0, pairs.shape[0], step):
+        chunk = pairs[i:i + step]
+        f.ingest_pairs(chunk)
+        l.ingest_pairs(chunk)
+        cf, cl = f.connected_components(), l.connected_components()
+        assert f.last_query_rounds == l.last_query_rounds
+        assert f.last_query_round_sizes() == l.last_query_round_sizes()
+        assert np.array_equal(cf.canonical_labels(), cl.canonical_labels())
+        c, lab = oracle.exact_cc(1500, pairs[:i + step])
+        assert cf.size() == c and np.array_equal(cf.canonical_labels(), lab)
+        assert l.last_query_finish_round == 0
+        if f.last_query_finish_round:
+            took_over += 1
+            assert 2 <= f.last_query_finish_round <= f.last_query_rounds
+    assert took_over >= 2, took_over
+
+
 def test_round_size_diagnostics():
     # per-round sizes of the last Borůvka query: one entry per round, at most n vertices regrouped,
     # and once a supernode is summed from its complement every later round walks only the vertices
