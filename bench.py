@@ -69,6 +69,9 @@ def prefix_trace(trace: np.ndarray, upto: int) -> np.ndarray:
 
 
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_update_tiles_chunglu_n262144_traffic.md")
+VALU_FILE = os.path.join(REPO, "profiles", "k_update_tiles_chunglu_n262144_valu.md")
+# wave-level VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4 cycles x 2.4 GHz
+VALU_PEAK_GIPS = 256 * 2.4
 
 
 def workload_key(args) -> str:
@@ -90,6 +93,22 @@ def measured_traffic(args):
         return None
     try:
         return float(kv["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def measured_valu(args):
+    """SQ_INSTS_VALU per k_update_tiles launch from the committed rocprofv3 --pmc summary, when it
+    was measured on this exact workload; otherwise None."""
+    try:
+        with open(VALU_FILE) as f:
+            kv = dict(line.split(":", 1) for line in f if line.startswith(("workload_key:", "valu_insts_per_launch:")))
+    except OSError:
+        return None
+    if kv.get("workload_key", "").strip() != workload_key(args):
+        return None
+    try:
+        return float(kv["valu_insts_per_launch"])
     except (KeyError, ValueError):
         return None
 
@@ -194,6 +213,14 @@ def main() -> int:
     br = prof_tot["boruvka"]
     br_gbs = br["alg_bytes"] / (br["total_ms"] / 1e3) / 1e9 if br["total_ms"] > 0 else 0.0
 
+    def valu_roof(args, up):
+        insts = measured_valu(args)
+        if insts is None or up["launches"] == 0 or up["total_ms"] <= 0:
+            return None
+        gips = insts / (up["total_ms"] / up["launches"] / 1e3) / 1e9
+        return {"insts_per_launch": insts, "achieved": gips, "peak": VALU_PEAK_GIPS,
+                "unit": "G wave-instructions/s", "frac": gips / VALU_PEAK_GIPS}
+
     result = {
         "metric": METRIC,
         "value": value,
@@ -233,6 +260,7 @@ def main() -> int:
             "launches": up["launches"],
             "avg_launch_ms": up["total_ms"] / max(1, up["launches"]),
             "alg_bytes_per_launch": up["alg_bytes"] / max(1, up["launches"]),
+            "valu": valu_roof(args, up),
         },
         "boruvka_rounds": {
             "launches": br["launches"],

@@ -136,17 +136,18 @@ __device__ inline uint32_t dsu_prio(uint32_t v, uint32_t seed) {
   h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
   return h;
 }
-__device__ inline uint32_t dsu_find(uint32_t* parent, uint32_t u) {
-  // path halving, dsu.h:145-152
-  uint32_t p = ld_acq(parent + u);
-  while (true) {
-    uint32_t gp = ld_acq(parent + p);
+// path halving, dsu.h:145-152, from u whose parent p is already loaded
+__device__ inline uint32_t dsu_find_from(uint32_t* parent, uint32_t u, uint32_t p) {
+  while (p != u) {
+    const uint32_t gp = ld_acq(parent + p);
     if (gp == p) return p;
     st_rlx(parent + u, gp);
     u = gp;
     p = ld_acq(parent + u);
   }
+  return u;
 }
+__device__ inline uint32_t dsu_find(uint32_t* parent, uint32_t u) { return dsu_find_from(parent, u, ld_acq(parent + u)); }
 // find for kernels in which no thread links roots (only compression writes, which keep every
 // pointer on the path to the root): cached loads are enough.
 __device__ inline uint32_t dsu_find_stable(uint32_t* parent, uint32_t u) {
@@ -164,8 +165,10 @@ constexpr uint32_t kNoLink = 0xFFFFFFFFu;
 // one, or kNoLink when a and b were already connected.
 __device__ inline uint32_t dsu_link(uint32_t* parent, uint32_t a, uint32_t b, uint32_t pseed) {
   while (true) {
-    a = dsu_find(parent, a);
-    b = dsu_find(parent, b);
+    // both first parents in one round trip; a root (the common case) needs no second load
+    const uint32_t a_up = ld_acq(parent + a), b_up = ld_acq(parent + b);
+    a = dsu_find_from(parent, a, a_up);
+    b = dsu_find_from(parent, b, b_up);
     if (a == b) return kNoLink;
     uint32_t pa = dsu_prio(a, pseed), pb = dsu_prio(b, pseed);
     if (pa == pb && a > b) { uint32_t t = a; a = b; b = t; }
@@ -699,14 +702,12 @@ __global__ void k_eager_union(const uint2* __restrict__ pairs, size_t count, uin
     }
     const uint32_t child = dsu_link(d.parent, p.x, p.y, d.prio_seed);
     merged[i] = child != kNoLink ? 1 : 0;
-    if (child != kNoLink) d.sf_edge[child] = concat_pairing(p.x, p.y);
+    if (child != kNoLink) {
+      const uint64_t key = concat_pairing(p.x, p.y);
+      d.sf_edge[child] = key;
+      sf_insert(d, key);  // into the forest set; the deletion check runs once the whole batch is merged
+    }
   }
-}
-__global__ void k_eager_sf_insert(const uint2* __restrict__ pairs, size_t count, const uint8_t* __restrict__ merged,
-                                  DsuDev d) {
-  if (ld_acq(d.flags + kFlagEagerInvalid)) return;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
-    if (merged[i]) sf_insert(d, concat_pairing(pairs[i].x, pairs[i].y));
 }
 __global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, const uint8_t* __restrict__ merged,
                                  DsuDev d) {
@@ -1730,7 +1731,6 @@ void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const Ds
   const uint2* p2 = reinterpret_cast<const uint2*>(pairs);
   const uint32_t grid = grid_for(count, 256);
   hipLaunchKernelGGL(k_eager_union, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
-  hipLaunchKernelGGL(k_eager_sf_insert, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
   hipLaunchKernelGGL(k_eager_sf_check, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
 }
 
