@@ -24,6 +24,7 @@ from ._lib import (  # noqa: F401
     OutOfSamplesException,
     Sketch,
     UpdateLockedException,
+    autotune,
     calc_bkt_per_col,
     calc_cc_samples,
     calc_vector_length,
@@ -47,6 +48,7 @@ __all__ = [
     "CONNECTIVITY",
     "KSPANNINGFORESTS",
     "calc_vector_length",
+    "autotune",
     "calc_cc_samples",
     "calc_bkt_per_col",
     "device_count",

@@ -53,6 +53,16 @@ class _AlgConfig(ctypes.Structure):
     ]
 
 
+class _AutotuneReport(ctypes.Structure):
+    """gz_autotune_report (include/gz_hip.h)."""
+    _fields_ = [
+        ("trials", ctypes.c_uint32),
+        ("tile", ctypes.c_uint32 * 16),
+        ("flush_threshold", ctypes.c_uint64 * 16),
+        ("ms", ctypes.c_double * 16),
+    ]
+
+
 class Request(ctypes.Structure):
     """gz_request (include/gz_serving.h)."""
     _fields_ = [
@@ -160,6 +170,7 @@ _SIGS = {
     "gz_alg_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_alg_set_query_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_alg_set_round_finish": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_autotune": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, _sz, _vp, _vp]),
     "gz_alg_last_query_finish_round": (ctypes.c_uint32, [_vp]),
     "gz_alg_last_query_round_sizes": (_sz, [_vp, _vp, _vp, _sz]),
     "gz_alg_get_profile": (ctypes.c_int, [_vp, ctypes.POINTER(AlgProfile)]),
@@ -368,6 +379,21 @@ class CCAlgConfiguration:
         c.tile = self.tile
         c.flush_threshold = self.flush_threshold
         return c
+
+
+def autotune(n: int, seed: int, pairs_device_ptr: int, count: int, config: CCAlgConfiguration | None = None):
+    """Tile size and flush threshold for this stream shape (gz_autotune): times the device ingest of
+    `count` (src,dst) pairs at `pairs_device_ptr` on scratch instances.  Returns the configuration
+    (the given one with the fastest pair) and the trials as (tile, flush_threshold, ms)."""
+    base = (config or CCAlgConfiguration()).to_c()
+    out = _AlgConfig()
+    rep = _AutotuneReport()
+    _check(lib().gz_autotune(n, seed, ctypes.byref(base), ctypes.c_void_p(pairs_device_ptr), count, ctypes.byref(out),
+                             ctypes.byref(rep)))
+    cfg = CCAlgConfiguration(out.sketches_factor, out.batch_factor, out.gamma_bytes, bool(out.eager_dsu),
+                             bool(out.batched), int(out.tile), int(out.flush_threshold))
+    trials = [(int(rep.tile[i]), int(rep.flush_threshold[i]), float(rep.ms[i])) for i in range(rep.trials)]
+    return cfg, trials
 
 
 def footprint_bytes(n: int, config: CCAlgConfiguration | None = None) -> int:

@@ -6,6 +6,7 @@
 #include "gz_runtime.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <fstream>
 
@@ -1209,6 +1210,63 @@ int gz_alg_set_query_graphs(gz_alg* a, int enable) {
   return guarded([&] {
     GZ_REQUIRE(a, "null alg");
     a->a->use_graphs = enable != 0;
+  });
+}
+int gz_autotune(uint32_t n, uint64_t seed, const gz_alg_config* base, const uint32_t* pairs_device, size_t count,
+                gz_alg_config* out, gz_autotune_report* report) {
+  return guarded([&] {
+    GZ_REQUIRE(out && pairs_device && count, "null argument or empty sample");
+    gz_alg_config b;
+    gz_alg_config_default(&b);
+    if (base) b = *base;
+    b.eager_dsu = 0;  // time the sketch update path only
+    b.batched = 1;
+    // candidates: tiles of 256..2048 destinations; flush thresholds of a quarter, half and all of
+    // the sample (at least 2^16 updates), so the sample is cut into 4, 2 or 1 launches
+    const uint32_t tiles[] = {256, 512, 1024, 2048};
+    std::vector<uint64_t> flushes;
+    for (uint64_t div : {4ull, 2ull, 1ull}) {
+      const uint64_t f = std::max<uint64_t>(uint64_t(1) << 16, ((uint64_t)count + div - 1) / div);
+      if (std::find(flushes.begin(), flushes.end(), f) == flushes.end()) flushes.push_back(f);
+    }
+    gz_autotune_report rep{};
+    double best = -1.0;
+    gz_alg_config pick = b;
+    for (uint64_t f : flushes)
+      for (uint32_t t : tiles) {
+        if (rep.trials >= GZ_AUTOTUNE_MAX_TRIALS) break;
+        gz_alg_config c = b;
+        c.tile = t;
+        c.flush_threshold = f;
+        GzAlg scratch(n, seed, c);
+        double ms = -1.0;
+        for (int rep_i = 0; rep_i < 3; ++rep_i) {  // a warm-up ingest, then two timed ones
+          const auto t0 = std::chrono::steady_clock::now();
+          scratch.ingest_device(pairs_device, count);
+          scratch.flush_all();  // synchronises
+          const double d = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+          if (rep_i > 0 && (ms < 0 || d < ms)) ms = d;
+        }
+        rep.tile[rep.trials] = t;
+        rep.flush_threshold[rep.trials] = f;
+        rep.ms[rep.trials] = ms;
+        ++rep.trials;
+        if (best < 0 || ms < best) {
+          best = ms;
+          pick = c;
+        }
+      }
+    if (base) {  // keep the caller's eager-DSU and form choices
+      pick.eager_dsu = base->eager_dsu;
+      pick.batched = base->batched;
+    } else {
+      gz_alg_config d;
+      gz_alg_config_default(&d);
+      pick.eager_dsu = d.eager_dsu;
+      pick.batched = d.batched;
+    }
+    *out = pick;
+    if (report) *report = rep;
   });
 }
 int gz_alg_set_round_finish(gz_alg* a, int enable) {

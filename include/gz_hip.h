@@ -183,6 +183,23 @@ int gz_alg_set_round_finish(gz_alg* a, int enable);
 /* No counterpart in the reference (diagnostics).  The Borůvka round from which the last query ran
  * its remaining rounds in the single-workgroup finish, or 0 when it did not. */
 uint32_t gz_alg_last_query_finish_round(const gz_alg* a);
+/* No counterpart in the reference (the update-tile and flush-threshold autotuner).  Times the
+ * device ingest of a sample stream (count undirected updates as (src,dst) pairs in device memory,
+ * as gz_alg_ingest_pairs_device takes them) on scratch instances with num_vertices vertices and the
+ * base configuration, for every candidate tile size (destinations per update wave, so also waves
+ * per vertex) and flush threshold (updates staged on the device per grouping and update launch).
+ * *out gets the base configuration with the fastest pair; the report lists every trial.  The
+ * caller's instances are not touched, and results never depend on the choice: it changes how the
+ * work is cut into launches, not what is computed.  Needs device memory for one more instance. */
+#define GZ_AUTOTUNE_MAX_TRIALS 16
+typedef struct {
+  uint32_t trials;
+  uint32_t tile[GZ_AUTOTUNE_MAX_TRIALS];
+  uint64_t flush_threshold[GZ_AUTOTUNE_MAX_TRIALS];
+  double ms[GZ_AUTOTUNE_MAX_TRIALS];  /* best of two timed ingests of the sample */
+} gz_autotune_report;
+int gz_autotune(uint32_t num_vertices, uint64_t seed, const gz_alg_config* base, const uint32_t* pairs_device,
+                size_t count, gz_alg_config* out, gz_autotune_report* report);
 /* Device footprint in bytes of an instance with these parameters (sketch slab + work buffers). */
 uint64_t gz_alg_footprint_bytes(uint32_t num_vertices, const gz_alg_config* cfg);
 

@@ -58,7 +58,11 @@ def test_rmat_2e17_queries_every_10m():
     n = 1 << scale
     say("rmat: generating")
     pairs, types, _ = gen.rmat_stream(scale, N, 0.1, 3, device="cuda")
-    alg = gz.CCSketchAlg(n, 77)
+    # tile size and flush threshold autotuned on the first 10M updates of the stream
+    cfg, trials = gz.autotune(n, 77, pairs.data_ptr(), 10_000_000)
+    say(f"rmat: autotuned tile {cfg.tile} flush threshold {cfg.flush_threshold} from {len(trials)} trials")
+    assert (cfg.tile, cfg.flush_threshold) in [(t, f) for t, f, _ in trials]
+    alg = gz.CCSketchAlg(n, 77, cfg)
     step = 10_000_000
     for q in range(10):
         lo, hi = q * step, (q + 1) * step
