@@ -271,7 +271,8 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   vstamp.assign(n, 0);
   vtiles.assign(n, 0);
   eager_cap = size_t(1) << 20;
-  hflags.alloc(std::max<size_t>(kNumFlags, geo.samples + 2));
+  hflags.alloc(std::max<size_t>(kNumFlags, geo.samples + 2) + kNumFlags);
+  hflags_err = std::max<size_t>(kNumFlags, geo.samples + 2);
   hsmall.alloc(16);
   hround.alloc(128);
   GZ_HIP(hipEventCreateWithFlags(&hround_ev, hipEventDisableTiming));
@@ -659,7 +660,8 @@ hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
 // once per group: first the graph prefix (round 0, rounds 1..K and a finish attempt), then groups
 // of 2 rounds, each followed by a finish attempt.  A finish that takes over runs every remaining
 // round, so at most one does per query.
-void GzAlg::boruvka() {
+bool GzAlg::boruvka(const std::function<void()>* speculate) {
+  bool answered = false;
   update_locked = true;
   try {
     const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
@@ -688,8 +690,10 @@ void GzAlg::boruvka() {
     uint32_t next = first + 1;
     uint32_t fin_r0 = first + 1;  // first round of the last finish attempt queued
     int64_t rounds = -1;
+    bool first_group = true;
     while (true) {
       GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)dsu.nmod * 4, hipMemcpyDeviceToHost, st));
+      if (first_group && speculate) (*speculate)();
       GZ_HIP(hipStreamSynchronize(st));
       // rounds launched so far, or as far as a finish that took over went
       const uint32_t upto = std::max(next - 1, hflags.p[dsu.nmod - 1]);
@@ -698,7 +702,11 @@ void GzAlg::boruvka() {
           rounds = r;
           break;
         }
-      if (rounds >= 0) break;
+      if (rounds >= 0) {
+        answered = first_group && speculate;
+        break;
+      }
+      first_group = false;
       if (upto >= last_round)
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
       const uint32_t hi = std::min(next + 1, last_round);
@@ -727,6 +735,7 @@ void GzAlg::boruvka() {
     throw;
   }
   update_locked = false;
+  return answered;
 }
 
 void GzAlg::eager_batch(const uint32_t* pairs, size_t count, uint8_t* merged) {
@@ -777,31 +786,42 @@ bool GzAlg::answer_queries(const uint32_t* us, const uint32_t* vs, size_t npoint
   std::lock_guard<std::mutex> lk(query_mtx);
   sync_eager_state();
   const bool cached = dsu_valid;
+  if (npoint) {
+    qa.ensure(npoint);
+    qb.ensure(npoint);
+    qout.ensure(npoint);
+  }
+  // the answer work: point queries, the component count, the error flags
+  const std::function<void()> enqueue_answers = [&] {
+    if (npoint) {
+      GZ_HIP(hipMemcpyAsync(qa.p, us, npoint * 4, hipMemcpyHostToDevice, st));
+      GZ_HIP(hipMemcpyAsync(qb.p, vs, npoint * 4, hipMemcpyHostToDevice, st));
+      launch_point_queries(dsu, qa.p, qb.p, (uint32_t)npoint, qout.p, st);
+      GZ_HIP(hipMemcpyAsync(point_out, qout.p, npoint, hipMemcpyDeviceToHost, st));
+    }
+    if (want_cc) {
+      launch_labels(dsu, labels.p, labels.p + n, st);
+      GZ_HIP(hipMemcpyAsync(hsmall.p, labels.p + n, 4, hipMemcpyDeviceToHost, st));
+    }
+    GZ_HIP(hipMemcpyAsync(hflags.p + hflags_err, flags.p, kNumFlags * 4, hipMemcpyDeviceToHost, st));
+  };
+  bool answered = false;
   if (!cached) {
     {
       std::lock_guard<std::mutex> g(stage_mtx);
       if (cfg.batched) submit_stage_locked();
       flush_pending();
     }
-    boruvka();
+    // The answers are enqueued with the first group of rounds; they stand when that group
+    // finished the query (the usual case), else they are enqueued again after the last round.
+    answered = boruvka(&enqueue_answers);
   }
-  if (npoint) {
-    qa.ensure(npoint);
-    qb.ensure(npoint);
-    qout.ensure(npoint);
-    GZ_HIP(hipMemcpyAsync(qa.p, us, npoint * 4, hipMemcpyHostToDevice, st));
-    GZ_HIP(hipMemcpyAsync(qb.p, vs, npoint * 4, hipMemcpyHostToDevice, st));
-    launch_point_queries(dsu, qa.p, qb.p, (uint32_t)npoint, qout.p, st);
-    GZ_HIP(hipMemcpyAsync(point_out, qout.p, npoint, hipMemcpyDeviceToHost, st));
+  if (!answered) {
+    enqueue_answers();
+    GZ_HIP(hipStreamSynchronize(st));
   }
-  if (want_cc) {
-    launch_labels(dsu, labels.p, labels.p + n, st);
-    GZ_HIP(hipMemcpyAsync(hsmall.p, labels.p + n, 4, hipMemcpyDeviceToHost, st));
-  }
-  GZ_HIP(hipMemcpyAsync(hflags.p, flags.p, kNumFlags * 4, hipMemcpyDeviceToHost, st));
-  GZ_HIP(hipStreamSynchronize(st));
   if (want_cc && ncc_out) *ncc_out = hsmall.p[0];
-  if (hflags.p[kFlagBadInput]) {
+  if (hflags.p[hflags_err + kFlagBadInput]) {
     launch_fill_u32(flags.p + kFlagBadInput, 0, 1, st);
     GZ_HIP(hipStreamSynchronize(st));
     throw GzError(GZ_ERR_INVALID_ARG, "ingested device pairs named a vertex >= num_vertices; those pairs were skipped");

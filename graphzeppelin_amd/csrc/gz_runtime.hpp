@@ -182,7 +182,8 @@ class GzAlg {
   DevBuf<uint8_t> eager_merged;
   hipEvent_t eager_ev = nullptr;
   size_t eager_used = 0, eager_cap;
-  PinBuf<uint32_t> hflags;  // >= kNumFlags and >= samples + 2 entries
+  PinBuf<uint32_t> hflags;  // device flags or round flags at 0, a query's error flags at hflags_err
+  size_t hflags_err = 0;
   PinBuf<uint32_t> hsmall;
 
   // kernel timing (gz_alg_set_profiling)
@@ -246,7 +247,11 @@ class GzAlg {
   void submit_eager_locked();
   void sync_eager_state();
   void ensure_answer();
-  void boruvka();
+  // Runs Borůvka.  `speculate`, when given, enqueues the caller's answer work (kernels and copies
+  // reading the DSU); it runs right after the first group of rounds, and boruvka() returns true when
+  // that group finished the query, so the work read the final DSU and one synchronisation served
+  // both; false means it has to be enqueued again.
+  bool boruvka(const std::function<void()>* speculate = nullptr);
   uint32_t read_flag(int slot);
 
   // Query prefix (DSU reset, round 0, rounds 1..K and a single-workgroup finish of the remaining
