@@ -745,34 +745,53 @@ __global__ void k_sf_build(DsuDev d) {
   }
 }
 
-// Round 0 (run_round_zero, 223-244): every vertex samples column 0 of its own sketch.  One wave
-// per vertex; lane l checks depth l, the first good depth is the lowest set bit of the ballot.
+// Round 0 (run_round_zero, 223-244): every vertex samples column 0 of its own sketch
+// (Sketch::sample, src/sketch.cpp:94-116).  A wave takes four vertices in groups of 16 lanes: lane
+// 16j + i (i < 15) checks depth i of vertex v0 + j and lane 16j + 15 its deterministic bucket, so
+// one hash per lane serves four vertices.  The first good depth of a vertex is almost always below
+// 15 (a bucket at depth d holds about deg / 2^(d+1) edges); a vertex with no good bucket there
+// (very high degree, or none at all) is finished by the whole wave over depths 15..D-1.
 template <class G>
 __global__ void __launch_bounds__(256)
 k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restrict__ samp,
                 uint32_t* __restrict__ flags, unsigned char* __restrict__ det_cpy) {
-  constexpr uint32_t V = 4;  // vertices per wave iteration: all their loads are issued first
-  const uint32_t lane = lane_id();
+  constexpr uint32_t V = 4, GL = kWave / V, GD = GL - 1;  // vertices per wave, lanes per vertex, depths
+  constexpr uint32_t RB = 8 + sizeof(G);
+  const uint32_t lane = lane_id(), i = lane % GL, base = lane - i;
+  const uint32_t nd = g.D < GD ? g.D : GD;  // depths checked in the grouped pass
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
   for (uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * V; v0 < g.n; v0 += waves * V) {
-    // records 0..D of a row: lane d < D depth d of column 0, lane D the deterministic bucket
-    uint64_t ra[V];
-    G rg[V];
-#pragma unroll
-    for (uint32_t k = 0; k < V; ++k) {
-      ra[k] = 0;
-      rg[k] = 0;
-      if (v0 + k < g.n && lane <= g.D)
-        ld_rec<G>(slab + (size_t)(v0 + k) * g.row + (size_t)lane * (8 + sizeof(G)), &ra[k], &rg[k]);
+    const uint32_t v = v0 + lane / GL;
+    uint64_t a = 0;
+    G gm = 0;
+    const bool live = v < g.n && (i < nd || i == GD);
+    if (live) ld_rec<G>(slab + (size_t)v * g.row + (size_t)(i == GD ? g.D : i) * RB, &a, &gm);
+    const uint64_t mask = __ballot(live && (G)xxh3_mix(xxh3_prep(a), g.chk_bf) == gm);
+    const uint32_t gmask = (uint32_t)(mask >> base) & ((1u << GL) - 1);
+    const uint32_t colm = gmask & ((1u << nd) - 1);
+    // every lane takes part in the shuffles; lane i == 0 of a group decides
+    const uint64_t det_a = shfl_u64(a, (int)(base + GD));
+    const G det_g = (G)shfl_u64((uint64_t)gm, (int)(base + GD));
+    const uint64_t col_a = shfl_u64(a, (int)(base + (colm ? __builtin_ctz(colm) : 0)));
+    bool undecided = false;
+    if (i == 0 && v < g.n) {
+      if (det_a == 0 && det_g == 0) samp[v] = kSampZero;
+      else if ((gmask >> GD) & 1) samp[v] = sample_record(kGood, det_a, g.n, flags);
+      else if (colm) samp[v] = sample_record(kGood, col_a, g.n, flags);
+      else if (g.D <= GD) samp[v] = kSampFail;
+      else undecided = true;
     }
-#pragma unroll
-    for (uint32_t k = 0; k < V; ++k) {
-      if (v0 + k >= g.n) break;
-      uint64_t idx;
-      const int code = wave_sample<G>(ra[k], rg[k], g.D, g.chk_bf, &idx);
-      if (lane == 0) samp[v0 + k] = sample_record(code, idx, g.n, flags);
-      // the deterministic bucket does not depend on the round: later rounds read this compact copy
-      if (lane == g.D) st_rec<G>(det_cpy + (size_t)(v0 + k) * (8 + sizeof(G)), ra[k], rg[k]);
+    // the deterministic bucket does not depend on the round: later rounds read this compact copy
+    if (i == GD && v < g.n) st_rec<G>(det_cpy + (size_t)v * RB, a, gm);
+    for (uint64_t und = __ballot(undecided); und; und &= und - 1) {  // uniform over the wave
+      const uint32_t u = v0 + (uint32_t)__builtin_ctzll(und) / GL;
+      const uint32_t d = GD + lane;  // D - GD <= 64 lanes for every supported n
+      uint64_t b = 0;
+      G bg = 0;
+      if (d < g.D) ld_rec<G>(slab + (size_t)u * g.row + (size_t)d * RB, &b, &bg);
+      const uint64_t m2 = __ballot(d < g.D && (G)xxh3_mix(xxh3_prep(b), g.chk_bf) == bg);
+      const uint64_t ia = m2 ? rd_lane_u64(b, __builtin_ctzll(m2)) : 0;
+      if (lane == 0) samp[u] = m2 ? sample_record(kGood, ia, g.n, flags) : kSampFail;
     }
   }
 }
