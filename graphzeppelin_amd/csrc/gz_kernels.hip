@@ -161,18 +161,28 @@ __device__ inline uint32_t dsu_find_stable(uint32_t* parent, uint32_t u) {
 }
 
 constexpr uint32_t kNoLink = 0xFFFFFFFFu;
-// CAS union ordered by priority (dsu.h:155-167).  Returns the root that was linked under the other
-// one, or kNoLink when a and b were already connected.
-__device__ inline uint32_t dsu_link(uint32_t* parent, uint32_t a, uint32_t b, uint32_t pseed) {
+// CAS union ordered by rank (dsu.h:155-167): the higher hint first (when hints are given), then the
+// higher priority, then the smaller id.  Any total order that stays fixed while links run
+// concurrently keeps the forest acyclic (a root is only linked under a root that ranks higher), so
+// the order may differ between launches.  Hints that rank high-degree vertices first let the many
+// merges into a hub go through the smaller roots' own parent words instead of all contending for
+// the hub's.  Returns the root that was linked under the other one, or kNoLink when a and b were
+// already connected.
+__device__ inline uint32_t dsu_link(uint32_t* parent, uint32_t a, uint32_t b, uint32_t pseed,
+                                    const unsigned char* hint = nullptr) {
   while (true) {
-    // both first parents in one round trip; a root (the common case) needs no second load
+    // both first parents (and hints) in one round trip; a root (the common case) needs no more
     const uint32_t a_up = ld_acq(parent + a), b_up = ld_acq(parent + b);
-    a = dsu_find_from(parent, a, a_up);
-    b = dsu_find_from(parent, b, b_up);
-    if (a == b) return kNoLink;
-    uint32_t pa = dsu_prio(a, pseed), pb = dsu_prio(b, pseed);
-    if (pa == pb && a > b) { uint32_t t = a; a = b; b = t; }
-    if (pa < pb) { uint32_t t = a; a = b; b = t; }
+    uint32_t ha = hint ? hint[a] : 0, hb = hint ? hint[b] : 0;
+    const uint32_t ra = dsu_find_from(parent, a, a_up), rb = dsu_find_from(parent, b, b_up);
+    if (ra == rb) return kNoLink;
+    if (hint && ra != a) ha = hint[ra];
+    if (hint && rb != b) hb = hint[rb];
+    a = ra;
+    b = rb;
+    const uint32_t pa = dsu_prio(a, pseed), pb = dsu_prio(b, pseed);
+    const bool a_wins = ha != hb ? ha > hb : pa != pb ? pa > pb : a < b;
+    if (!a_wins) { uint32_t t = a; a = b; b = t; }
     uint32_t expected = b;
     if (__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT))
@@ -754,7 +764,7 @@ __global__ void k_sf_build(DsuDev d) {
 template <class G>
 __global__ void __launch_bounds__(256)
 k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restrict__ samp,
-                uint32_t* __restrict__ flags, unsigned char* __restrict__ det_cpy) {
+                uint32_t* __restrict__ flags, unsigned char* __restrict__ det_cpy, unsigned char* __restrict__ hint) {
   constexpr uint32_t V = 4, GL = kWave / V, GD = GL - 1;  // vertices per wave, lanes per vertex, depths
   constexpr uint32_t RB = 8 + sizeof(G);
   const uint32_t lane = lane_id(), i = lane % GL, base = lane - i;
@@ -769,12 +779,17 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
     const uint64_t mask = __ballot(live && (G)xxh3_mix(xxh3_prep(a), g.chk_bf) == gm);
     const uint32_t gmask = (uint32_t)(mask >> base) & ((1u << GL) - 1);
     const uint32_t colm = gmask & ((1u << nd) - 1);
+    // degree class: one past the deepest nonzero bucket among the grouped depths (a bucket at depth
+    // d is nonzero about when the degree reaches 2^d)
+    const uint64_t nzm = __ballot(live && i < nd && (a | (uint64_t)gm) != 0);
+    const uint32_t gnz = (uint32_t)(nzm >> base) & ((1u << nd) - 1);
     // every lane takes part in the shuffles; lane i == 0 of a group decides
     const uint64_t det_a = shfl_u64(a, (int)(base + GD));
     const G det_g = (G)shfl_u64((uint64_t)gm, (int)(base + GD));
     const uint64_t col_a = shfl_u64(a, (int)(base + (colm ? __builtin_ctz(colm) : 0)));
     bool undecided = false;
     if (i == 0 && v < g.n) {
+      hint[v] = (unsigned char)(gnz ? 32 - __builtin_clz(gnz) : 0);
       if (det_a == 0 && det_g == 0) samp[v] = kSampZero;
       else if ((gmask >> GD) & 1) samp[v] = sample_record(kGood, det_a, g.n, flags);
       else if (colm) samp[v] = sample_record(kGood, col_a, g.n, flags);
@@ -798,7 +813,9 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
 
 // The merges of a round (sample_supernode, src/cc_sketch_alg.cpp:137-166), one thread per
 // supernode: every vertex that was a root when the round began (root == nullptr: every vertex, as
-// in round 0) acts on its sample record.  The round is modified by a FAIL or a merge.
+// in round 0) acts on its sample record.  The round is modified by a FAIL or a merge.  Links rank
+// roots by the degree class round 0 recorded (hint), so hubs stay roots and absorb their
+// neighbours without contention.
 constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot, no giant)
 
 // Round state words (RoundBufs::totals).  A query starts with kStGiantFirst = kNoRoot.
@@ -828,11 +845,11 @@ __device__ inline uint32_t walk_vertex(const uint32_t* act_list, uint32_t round,
   return round >= 2 ? act_list[i] : i;
 }
 
-__device__ inline bool apply_sample(const DsuDev& d, uint64_t s) {
+__device__ inline bool apply_sample(const DsuDev& d, uint64_t s, const unsigned char* hint = nullptr) {
   if (s == kSampZero) return false;
   if (s == kSampFail) return true;
   const uint32_t a = (uint32_t)(s >> 32), b = (uint32_t)s;
-  const uint32_t child = dsu_link(d.parent, a, b, d.prio_seed);
+  const uint32_t child = dsu_link(d.parent, a, b, d.prio_seed, hint);
   if (child == kNoLink) return false;
   // A merge records its edge in the spanning forest (cc_sketch_alg.cpp:152-162), in the slot of
   // the root it linked (every root is linked at most once between resets, so slots never
@@ -847,7 +864,8 @@ __device__ inline bool apply_sample(const DsuDev& d, uint64_t s) {
 __global__ void __launch_bounds__(256)
 k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ root, DsuDev d,
                 const uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
-                const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod) {
+                const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod,
+                const unsigned char* __restrict__ hint) {
   const uint32_t live = prev_mod ? ld_acq(prev_mod) : 1u;  // checked after the first reads
   const bool act = st && st[kStActMode] != 0;
   const uint32_t gr = act ? st[kStGiantRound] : kNoRoot;
@@ -858,9 +876,9 @@ k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ 
     const uint32_t v = walk_vertex(act_list, round, i);
     if (root && root[v] != v) continue;
     if (v == gr) continue;  // handled once below
-    if (apply_sample(d, samp[v])) modified = true;
+    if (apply_sample(d, samp[v], hint)) modified = true;
   }
-  if (gr != kNoRoot && blockIdx.x == 0 && threadIdx.x == 0 && apply_sample(d, samp[gr])) modified = true;
+  if (gr != kNoRoot && blockIdx.x == 0 && threadIdx.x == 0 && apply_sample(d, samp[gr], hint)) modified = true;
   if (__syncthreads_or(modified) && threadIdx.x == 0) *cur_mod = 1;  // one plain store per block
 }
 
@@ -1351,7 +1369,8 @@ template <class G>
 __global__ void __launch_bounds__(kRedBlock)
 k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ act0,
             const unsigned char* __restrict__ slab, Geo g, uint32_t r0, uint32_t last_round,
-            uint64_t* __restrict__ samp, uint32_t* __restrict__ rstats, const unsigned char* __restrict__ det_cpy) {
+            uint64_t* __restrict__ samp, uint32_t* __restrict__ rstats, const unsigned char* __restrict__ det_cpy,
+            const unsigned char* __restrict__ hint) {
   __shared__ uint32_t s_key[kFinishTable], s_cnt[kFinishTable];
   __shared__ uint32_t s_mem[kFinishMax], s_mroot[kFinishMax], s_seg[kFinishMax];
   __shared__ uint64_t s_acc[kFinishChunks * 2 * kWave];
@@ -1496,8 +1515,8 @@ k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ ac
     // merges (k_apply_samples): every supernode root of the walk, and the giant
     bool modified = false;
     for (uint32_t k = tid; k < kFinishTable; k += kRedBlock)
-      if (s_key[k] != kEmptyKey && apply_sample(d, ld_acq(samp + s_key[k]))) modified = true;
-    if (tid == 0 && apply_sample(d, ld_acq(samp + gcur))) modified = true;
+      if (s_key[k] != kEmptyKey && apply_sample(d, ld_acq(samp + s_key[k]), hint)) modified = true;
+    if (tid == 0 && apply_sample(d, ld_acq(samp + gcur), hint)) modified = true;
     const bool any = __syncthreads_or(modified);
     if (tid == 0 && any) st_rlx(d.mod + r, 1u);
     if (!any || r >= last_round) {
@@ -1768,11 +1787,13 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
   uint64_t* samp = rb.samp;
   const uint32_t grid = grid_for((size_t)(g.n + 3) / 4 * kWave, 256, 16384);
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy);
+    hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy,
+                       rb.hint);
   else
-    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy);
+    hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy,
+                       rb.hint);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr, 0u,
-                     nullptr, nullptr, d.mod);
+                     nullptr, nullptr, d.mod, rb.hint);
 }
 
 // Host bound on each level's entry count (the kernels compute the exact counts on the device):
@@ -1804,7 +1825,7 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
     hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(reduce_blocks(n)), dim3(kRedBlock), 0, s, st, members, rb.root,
                        slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
-                     act_list, prev, cur);
+                     act_list, prev, cur, rb.hint);
 }
 
 void launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned char* slab, const Geo& g,
@@ -1813,10 +1834,10 @@ void launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned 
   const uint32_t* act0 = rb.members[(r0 - 1) & 1];
   if (gbytes == 4)
     hipLaunchKernelGGL(k_bv_finish<uint32_t>, dim3(1), dim3(kRedBlock), 0, s, d, rb.totals, act0, slab, g, r0,
-                       last_round, rb.samp, rb.rstats, rb.det_cpy);
+                       last_round, rb.samp, rb.rstats, rb.det_cpy, rb.hint);
   else
     hipLaunchKernelGGL(k_bv_finish<uint64_t>, dim3(1), dim3(kRedBlock), 0, s, d, rb.totals, act0, slab, g, r0,
-                       last_round, rb.samp, rb.rstats, rb.det_cpy);
+                       last_round, rb.samp, rb.rstats, rb.det_cpy, rb.hint);
 }
 uint32_t finish_max() { return kFinishMax; }
 

@@ -92,6 +92,7 @@ struct RoundBufs {
   uint64_t* samp;        // n      sample record per supernode root
   uint32_t* rstats;      // 2 * 64 per round: vertices regrouped, giant root (diagnostics)
   unsigned char* det_cpy;  // n records: every vertex's deterministic bucket, copied in round 0
+  unsigned char* hint;     // n      round 0: each vertex's degree class, the first key of its merge order
 };
 // Writes rb.samp: a sample record per vertex.
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,

@@ -173,9 +173,9 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ + part +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
   // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
-  // forest edge (8 each), spanning-forest hash set (16), deterministic-bucket copy; the reduce's
-  // cross-workgroup accumulators (1 KiB + 4 B per 1024 vertices, and one more slot)
-  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 16 + 8 + gbytes) + ((uint64_t)n / 1024 + 2) * 1028 + 8192;
+  // forest edge (8 each), spanning-forest hash set (16), deterministic-bucket copy, degree class
+  // (1); the reduce's cross-workgroup accumulators (1 KiB + 4 B per 1024 vertices, and one more slot)
+  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 16 + 8 + gbytes + 1) + ((uint64_t)n / 1024 + 2) * 1028 + 8192;
   return slab + ingest + rounds;
 }
 
@@ -230,6 +230,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   r_totals.alloc(8);
   r_stats.alloc(128);
   r_det.alloc((size_t)n * (8 + gb));
+  r_hint.alloc(n);
   launch_fill_u32(r_stats.p, 0, 128, st);
   // cross-workgroup accumulators of the reduce: one slot per workgroup and one for the giant
   const size_t xslots = (size_t)reduce_blocks(n) + 1;
@@ -238,7 +239,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   launch_fill_u64(r_xacc.p, 0, xslots * 128, st);
   launch_fill_u32(r_xcnt.p, 0, xslots, st);
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
-                 r_xacc.p, r_xcnt.p, r_samp.p, r_stats.p, r_det.p};
+                 r_xacc.p, r_xcnt.p, r_samp.p, r_stats.p, r_det.p, r_hint.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest

@@ -144,7 +144,7 @@ class GzAlg {
   // Borůvka round buffers
   DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_totals, r_stats, mod;
   DevBuf<uint64_t> r_xacc, r_samp;
-  DevBuf<uint8_t> r_det;
+  DevBuf<uint8_t> r_det, r_hint;
   DevBuf<uint32_t> r_xcnt;
   RoundBufs rb{};
   // device stream ingest (pending pairs -> grouping -> tiles)
