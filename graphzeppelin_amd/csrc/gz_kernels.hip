@@ -914,9 +914,11 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
   uint32_t gfirst = st[kStGiantFirst];
   const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t walk = walk_count(st, round, n);
+  const bool in_walk = i < walk;
   const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
-  if (!live) return;
+  // the grid covers n; blocks past the walk have nothing to count (block 0 writes the round state)
+  if (!live || (blockIdx.x > 0 && blockIdx.x * kAggBlock >= walk)) return;
   __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable];
   for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock) {
     t_key[k] = kEmptyKey;
@@ -962,10 +964,11 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
   const bool act = st[kStActMode] != 0;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t walk = walk_count(st, round, n);
+  const bool in_walk = i < walk;
   const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
   uint32_t c = in_walk ? cnt[v] : 0;
-  if (!live) return;
+  if (!live || blockIdx.x * kAggBlock >= walk) return;  // whole blocks past the walk
   if (c) cnt[v] = 0;
   if (!act && giant_ok && 2ull * c > n) {
     st[kStGiantFirst] = v;
@@ -998,10 +1001,11 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads (see k_bv_find_count)
   const uint32_t gr = st[kStGiantRound];
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
-  const bool in_walk = i < walk_count(st, round, n);
+  const uint32_t walk = walk_count(st, round, n);
+  const bool in_walk = i < walk;
   const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
   const uint32_t r = in_walk ? root[v] : 0;
-  if (!live) return;
+  if (!live || blockIdx.x * kAggBlock >= walk) return;  // whole blocks past the walk
   __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable], t_base[kAggTable];
   for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock) {
     t_key[k] = kEmptyKey;
