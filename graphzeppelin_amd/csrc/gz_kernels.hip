@@ -1557,6 +1557,18 @@ __global__ void k_point_queries(uint32_t* __restrict__ parent, const uint32_t* _
     out[i] = dsu_find(parent, qa[i]) == dsu_find(parent, qb[i]) ? 1 : 0;
 }
 
+// The point queries of a query group and the error flags, read from and written to mapped host
+// memory directly (a few words, so no copy launches around the kernel).
+__global__ void k_answer_points(uint32_t* __restrict__ parent, const uint32_t* __restrict__ flags,
+                                const uint32_t* __restrict__ q, uint32_t nq, uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < kNumFlags) out[t] = flags[t];
+  uint8_t* ans = reinterpret_cast<uint8_t*>(out + kNumFlags);
+  for (uint32_t i = t; i < nq; i += gridDim.x * blockDim.x)
+    ans[i] = dsu_find(parent, q[i]) == dsu_find(parent, q[nq + i]) ? 1 : 0;
+  __threadfence_system();
+}
+
 // ================================================================================================
 // Standalone sketch ops
 // ================================================================================================
@@ -1874,6 +1886,10 @@ void launch_digest(int gbytes, const unsigned char* slab, const Geo& g, uint64_t
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s) {
   launch_fill_u32(ncc, 0, 1, s);
   hipLaunchKernelGGL(k_labels, dim3(grid_for(d.n, 256)), dim3(256), 0, s, d.parent, d.n, labels, ncc);
+}
+void launch_answer_points(const DsuDev& d, const uint32_t* q, uint32_t nq, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_answer_points, dim3(grid_for(nq > kNumFlags ? nq : kNumFlags, 256)), dim3(256), 0, s, d.parent,
+                     d.flags, q, nq, out);
 }
 void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,
                           hipStream_t s) {

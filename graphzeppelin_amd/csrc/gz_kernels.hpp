@@ -115,6 +115,9 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
 void launch_sf_build(const DsuDev& d, hipStream_t s);
 void launch_round_start(const RoundBufs& rb, hipStream_t s);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
+// A query group's answers in place in mapped host memory: q[0..nq) and q[nq..2nq) are the pairs;
+// out gets the kNumFlags error flags, then one byte per pair (1: connected).
+void launch_answer_points(const DsuDev& d, const uint32_t* q, uint32_t nq, uint32_t* out, hipStream_t s);
 void launch_point_queries(const DsuDev& d, const uint32_t* qa, const uint32_t* qb, uint32_t nq, uint8_t* out,
                           hipStream_t s);
 

@@ -272,8 +272,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   vstamp.assign(n, 0);
   vtiles.assign(n, 0);
   eager_cap = size_t(1) << 20;
-  hflags.alloc(std::max<size_t>(kNumFlags, geo.samples + 2) + kNumFlags);
-  hflags_err = std::max<size_t>(kNumFlags, geo.samples + 2);
+  hflags.alloc(std::max<size_t>(kNumFlags, geo.samples + 2));
   hsmall.alloc(16);
   hround.alloc(128);
   GZ_HIP(hipEventCreateWithFlags(&hround_ev, hipEventDisableTiming));
@@ -787,24 +786,22 @@ bool GzAlg::answer_queries(const uint32_t* us, const uint32_t* vs, size_t npoint
   std::lock_guard<std::mutex> lk(query_mtx);
   sync_eager_state();
   const bool cached = dsu_valid;
+  // The pairs go to mapped host memory, where the answer kernel reads them and leaves the error
+  // flags and the answers (no copy launches around it).  The previous group's kernel has finished:
+  // every group ends with a stream synchronisation.
+  aq_in.ensure(std::max<size_t>(1, 2 * npoint));
+  aq_out.ensure(kNumFlags + (npoint + 3) / 4);
   if (npoint) {
-    qa.ensure(npoint);
-    qb.ensure(npoint);
-    qout.ensure(npoint);
+    std::memcpy(aq_in.p, us, npoint * 4);
+    std::memcpy(aq_in.p + npoint, vs, npoint * 4);
   }
   // the answer work: point queries, the component count, the error flags
   const std::function<void()> enqueue_answers = [&] {
-    if (npoint) {
-      GZ_HIP(hipMemcpyAsync(qa.p, us, npoint * 4, hipMemcpyHostToDevice, st));
-      GZ_HIP(hipMemcpyAsync(qb.p, vs, npoint * 4, hipMemcpyHostToDevice, st));
-      launch_point_queries(dsu, qa.p, qb.p, (uint32_t)npoint, qout.p, st);
-      GZ_HIP(hipMemcpyAsync(point_out, qout.p, npoint, hipMemcpyDeviceToHost, st));
-    }
+    launch_answer_points(dsu, aq_in.d, (uint32_t)npoint, aq_out.d, st);
     if (want_cc) {
       launch_labels(dsu, labels.p, labels.p + n, st);
       GZ_HIP(hipMemcpyAsync(hsmall.p, labels.p + n, 4, hipMemcpyDeviceToHost, st));
     }
-    GZ_HIP(hipMemcpyAsync(hflags.p + hflags_err, flags.p, kNumFlags * 4, hipMemcpyDeviceToHost, st));
   };
   bool answered = false;
   if (!cached) {
@@ -822,7 +819,8 @@ bool GzAlg::answer_queries(const uint32_t* us, const uint32_t* vs, size_t npoint
     GZ_HIP(hipStreamSynchronize(st));
   }
   if (want_cc && ncc_out) *ncc_out = hsmall.p[0];
-  if (hflags.p[hflags_err + kFlagBadInput]) {
+  if (npoint) std::memcpy(point_out, reinterpret_cast<const uint8_t*>(aq_out.p + kNumFlags), npoint);
+  if (aq_out.p[kFlagBadInput]) {
     launch_fill_u32(flags.p + kFlagBadInput, 0, 1, st);
     GZ_HIP(hipStreamSynchronize(st));
     throw GzError(GZ_ERR_INVALID_ARG, "ingested device pairs named a vertex >= num_vertices; those pairs were skipped");

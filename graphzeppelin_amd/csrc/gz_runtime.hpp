@@ -88,6 +88,31 @@ struct PinBuf {
   }
 };
 
+// Owning pinned host buffer that kernels read and write in place (fine-grained, mapped): for a
+// few words per query, a kernel access over the bus replaces a copy launch.
+template <class T>
+struct MapBuf {
+  T* p = nullptr;  // host address
+  T* d = nullptr;  // device address
+  size_t n = 0;
+  MapBuf() = default;
+  MapBuf(const MapBuf&) = delete;
+  MapBuf& operator=(const MapBuf&) = delete;
+  ~MapBuf() { release(); }
+  void ensure(size_t count) {
+    if (count <= n) return;
+    release();
+    GZ_HIP(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));
+    GZ_HIP(hipHostGetDevicePointer((void**)&d, p, 0));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = d = nullptr;
+    n = 0;
+  }
+};
+
 int device_cu_count();
 void require_device();
 
@@ -182,9 +207,10 @@ class GzAlg {
   DevBuf<uint8_t> eager_merged;
   hipEvent_t eager_ev = nullptr;
   size_t eager_used = 0, eager_cap;
-  PinBuf<uint32_t> hflags;  // device flags or round flags at 0, a query's error flags at hflags_err
-  size_t hflags_err = 0;
+  PinBuf<uint32_t> hflags;  // device flags, or a query's round flags
   PinBuf<uint32_t> hsmall;
+  MapBuf<uint32_t> aq_in;  // a query group's point queries: us then vs, read by the answer kernel
+  MapBuf<uint32_t> aq_out;  // the error flags, then one answer byte per point query, written by it
 
   // kernel timing (gz_alg_set_profiling)
   struct Timed {
