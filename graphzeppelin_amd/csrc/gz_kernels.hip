@@ -701,10 +701,17 @@ inline uint32_t grid_for(size_t work, uint32_t block, uint32_t cap = 8192) {
 // reference's stream threads are concurrent as well): a deleted forest edge always invalidates,
 // and any edge that merged and occurs again in the batch invalidates.
 // ================================================================================================
-__global__ void k_eager_union(const uint2* __restrict__ pairs, size_t count, uint8_t* __restrict__ merged, DsuDev d) {
-  if (ld_acq(d.flags + kFlagEagerInvalid)) return;
+// src != pairs: the batch is also copied from src into pairs (the staging buffer) on the way, so
+// ingesting device pairs needs no separate copy launch; the copy happens whether or not the DSU
+// is still valid.
+__global__ void k_eager_union(const uint2* __restrict__ src, uint2* __restrict__ pairs, size_t count,
+                              uint8_t* __restrict__ merged, DsuDev d) {
+  const bool live = !ld_acq(d.flags + kFlagEagerInvalid);
+  if (!live && src == pairs) return;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
-    const uint2 p = pairs[i];
+    const uint2 p = src[i];
+    if (src != pairs) pairs[i] = p;
+    if (!live) continue;
     if (p.x >= d.n || p.y >= d.n) {
       merged[i] = 0;
       st_rlx(d.flags + kFlagBadInput, 1u);
@@ -1780,11 +1787,13 @@ void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile
 
 size_t part_hist_len() { return (size_t)kPartBins * kPartMaxBlocks + 1; }
 
-void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d, hipStream_t s) {
+void launch_eager(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d,
+                  hipStream_t s) {
   if (!count) return;
-  const uint2* p2 = reinterpret_cast<const uint2*>(pairs);
+  uint2* p2 = reinterpret_cast<uint2*>(pairs);
   const uint32_t grid = grid_for(count, 256);
-  hipLaunchKernelGGL(k_eager_union, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
+  hipLaunchKernelGGL(k_eager_union, dim3(grid), dim3(256), 0, s, reinterpret_cast<const uint2*>(src), p2, count, merged,
+                     d);
   hipLaunchKernelGGL(k_eager_sf_check, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
 }
 

@@ -76,7 +76,9 @@ void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile
                   const FlushBytes* fb = nullptr);
 
 // ---- eager DSU (batch form of pre_insert, src/cc_sketch_alg.cpp:79-104) ----
-void launch_eager(const uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d, hipStream_t s);
+// src != pairs: the batch is copied from src into pairs by the same launch.
+void launch_eager(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d,
+                  hipStream_t s);
 
 // ---- Borůvka ----
 void launch_dsu_reset(const DsuDev& d, hipStream_t s);

@@ -320,7 +320,7 @@ void GzAlg::pre_insert(uint32_t src, uint32_t dst) {
 void GzAlg::submit_eager_locked() {
   if (!eager_used) return;
   GZ_HIP(hipMemcpyAsync(eager_dev.p, eager_host.p, eager_used * 8, hipMemcpyHostToDevice, st));
-  eager_batch(eager_dev.p, eager_used, eager_merged.p);
+  eager_batch(eager_dev.p, eager_dev.p, eager_used, eager_merged.p);
   GZ_HIP(hipGetLastError());
   GZ_HIP(hipEventRecord(eager_ev, st));
   GZ_HIP(hipEventSynchronize(eager_ev));  // the host buffer is reused right away
@@ -522,8 +522,8 @@ void GzAlg::ingest_device(const uint32_t* pairs_dev, size_t count) {
   while (count) {
     const size_t take = std::min(count, cap_pairs - pending_n);
     uint32_t* dst = pending.p + 2 * pending_n;
-    GZ_HIP(hipMemcpyAsync(dst, pairs_dev, take * 8, hipMemcpyDeviceToDevice, st));
-    if (cfg.eager_dsu && dsu_valid) eager_batch(dst, take, merged.p);
+    if (cfg.eager_dsu && dsu_valid) eager_batch(pairs_dev, dst, take, merged.p);  // copies as well
+    else GZ_HIP(hipMemcpyAsync(dst, pairs_dev, take * 8, hipMemcpyDeviceToDevice, st));
     pending_n += take;
     pairs_dev += 2 * take;
     count -= take;
@@ -541,7 +541,7 @@ void GzAlg::ingest_host(const uint32_t* pairs, size_t count) {
     uint32_t* dst = pending.p + 2 * pending_n;
     GZ_HIP(hipMemcpyAsync(dst, pairs, take * 8, hipMemcpyHostToDevice, st));
     GZ_HIP(hipStreamSynchronize(st));
-    if (cfg.eager_dsu && dsu_valid) eager_batch(dst, take, merged.p);
+    if (cfg.eager_dsu && dsu_valid) eager_batch(dst, dst, take, merged.p);
     pending_n += take;
     pairs += 2 * take;
     count -= take;
@@ -738,12 +738,12 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
   return answered;
 }
 
-void GzAlg::eager_batch(const uint32_t* pairs, size_t count, uint8_t* merged) {
+void GzAlg::eager_batch(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* merged) {
   if (sf_pending) {
     launch_sf_build(dsu, st);
     sf_pending = false;
   }
-  launch_eager(pairs, count, merged, dsu, st);
+  launch_eager(src, pairs, count, merged, dsu, st);
 }
 
 // connected_components, src/cc_sketch_alg.cpp:515-557 + ConnectedComponents, return_types.cpp:5-19

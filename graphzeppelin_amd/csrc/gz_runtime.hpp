@@ -300,7 +300,8 @@ class GzAlg {
   // The spanning-forest membership set that the eager DSU's deletion check reads is built from the
   // forest edges after a query, but only before the first eager batch that needs it.
   bool sf_pending = false;
-  void eager_batch(const uint32_t* pairs, size_t count, uint8_t* merged);
+  // src != pairs: the batch is copied from src into pairs (device memory) by the same launch
+  void eager_batch(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* merged);
   void enqueue_query_prefix(bool giant_ok, uint32_t last);
   hipGraphExec_t get_query_graph(bool giant_ok, uint32_t last);
 };
