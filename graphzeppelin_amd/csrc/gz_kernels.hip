@@ -1381,7 +1381,7 @@ __global__ void __launch_bounds__(kRedBlock)
 k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ act0,
             const unsigned char* __restrict__ slab, Geo g, uint32_t r0, uint32_t last_round,
             uint64_t* __restrict__ samp, uint32_t* __restrict__ rstats, const unsigned char* __restrict__ det_cpy,
-            const unsigned char* __restrict__ hint) {
+            const unsigned char* __restrict__ hint, uint32_t* __restrict__ rep) {
   __shared__ uint32_t s_key[kFinishTable], s_cnt[kFinishTable];
   __shared__ uint32_t s_mem[kFinishMax], s_mroot[kFinishMax], s_seg[kFinishMax];
   __shared__ uint64_t s_acc[kFinishChunks * 2 * kWave];
@@ -1402,8 +1402,20 @@ k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ ac
     s_gfirst = gfirst;
     s_walk = walk;
   }
+  // The last kernel of every group of rounds: on exit it copies the round flags and the per-round
+  // diagnostics to rep (mapped host memory), where the host reads them after its wait, so no copy
+  // launch follows the group.
+  auto report = [&]() {
+    __syncthreads();
+    for (uint32_t t = tid; t < d.nmod; t += kRedBlock) rep[t] = ld_acq(d.mod + t);
+    for (uint32_t t = tid; t < 2 * kMaxStatRounds; t += kRedBlock) rep[d.nmod + t] = ld_acq(rstats + t);
+    __threadfence_system();
+  };
   __syncthreads();
-  if (!s_take) return;
+  if (!s_take) {
+    if (rep) report();
+    return;
+  }
   uint32_t walk = s_walk;
   uint32_t v[kFinishPer];
 #pragma unroll
@@ -1532,6 +1544,7 @@ k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ ac
     if (tid == 0 && any) st_rlx(d.mod + r, 1u);
     if (!any || r >= last_round) {
       if (tid == 0) st_rlx(d.mod + (d.nmod - 1), r);
+      if (rep) report();
       return;
     }
     // the next round walks this round's members
@@ -1853,16 +1866,17 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
                      act_list, prev, cur, rb.hint);
 }
 
-void launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned char* slab, const Geo& g,
-                   const DsuDev& d, const RoundBufs& rb, hipStream_t s) {
-  if (r0 < 2 || r0 > last_round) return;  // round r0 walks round r0 - 1's member list
+bool launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned char* slab, const Geo& g,
+                   const DsuDev& d, const RoundBufs& rb, hipStream_t s, uint32_t* rep) {
+  if (r0 < 2 || r0 > last_round) return false;  // round r0 walks round r0 - 1's member list
   const uint32_t* act0 = rb.members[(r0 - 1) & 1];
   if (gbytes == 4)
     hipLaunchKernelGGL(k_bv_finish<uint32_t>, dim3(1), dim3(kRedBlock), 0, s, d, rb.totals, act0, slab, g, r0,
-                       last_round, rb.samp, rb.rstats, rb.det_cpy, rb.hint);
+                       last_round, rb.samp, rb.rstats, rb.det_cpy, rb.hint, rep);
   else
     hipLaunchKernelGGL(k_bv_finish<uint64_t>, dim3(1), dim3(kRedBlock), 0, s, d, rb.totals, act0, slab, g, r0,
-                       last_round, rb.samp, rb.rstats, rb.det_cpy, rb.hint);
+                       last_round, rb.samp, rb.rstats, rb.det_cpy, rb.hint, rep);
+  return true;
 }
 uint32_t finish_max() { return kFinishMax; }
 

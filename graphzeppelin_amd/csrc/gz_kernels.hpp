@@ -104,8 +104,10 @@ uint32_t reduce_blocks(uint32_t n);
 // Rounds r0, r0 + 1, ... up to last_round in one workgroup, when round r0 - 1 changed something, a
 // giant supernode exists and round r0 walks at most finish_max() vertices; otherwise a no-op.
 // d.mod[d.nmod - 1] gets the last round it ran (it stays 0 when the finish did not take over).
-void launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned char* slab, const Geo& g,
-                   const DsuDev& d, const RoundBufs& rb, hipStream_t s);
+// rep (mapped host memory, or nullptr): on exit the kernel copies d.mod[0..nmod) and the 128
+// per-round diagnostic words there.  Returns whether it launched (2 <= r0 <= last_round).
+bool launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned char* slab, const Geo& g,
+                   const DsuDev& d, const RoundBufs& rb, hipStream_t s, uint32_t* rep = nullptr);
 uint32_t finish_max();
 // Queue round `round` (>= 1); every kernel of it is a no-op when d.mod[round - 1] == 0.
 // giant_ok: the vertex sketches are known to XOR to zero, so a supernode holding more than half of

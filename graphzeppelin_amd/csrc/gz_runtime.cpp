@@ -275,6 +275,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   hflags.alloc(std::max<size_t>(kNumFlags, geo.samples + 2));
   hsmall.alloc(16);
   hround.alloc(128);
+  rep.ensure(dsu.nmod + 128);
   GZ_HIP(hipEventCreateWithFlags(&hround_ev, hipEventDisableTiming));
   GZ_HIP(hipEventCreateWithFlags(&eager_ev, hipEventDisableTiming));
   GZ_HIP(hipStreamSynchronize(st));
@@ -630,7 +631,7 @@ void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
   launch_round_start(rb, st);
   launch_sample_round0(gb, slab.p, geo, dsu, rb, st);
   for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
-  if (giant_ok && use_finish) launch_finish(gb, last + 1, geo.samples - 1, slab.p, geo, dsu, rb, st);
+  if (giant_ok && use_finish) launch_finish(gb, last + 1, geo.samples - 1, slab.p, geo, dsu, rb, st, rep.d);
 }
 
 hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
@@ -671,7 +672,7 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
       // The launched rounds of this query: up to the first round of the previous query after which
       // a giant existed and at most finish_max() vertices remained outside it (diagnostics: vertices
       // regrouped and the giant's root per round).
-      GZ_HIP(hipEventSynchronize(hround_ev));
+      if (hround_on_event) GZ_HIP(hipEventSynchronize(hround_ev));
       hround_pending = false;
       next_prefix = (uint32_t)last_rounds + 1;
       for (uint32_t r = 1; r < (uint32_t)last_rounds && r < 64; ++r)
@@ -691,14 +692,19 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
     uint32_t fin_r0 = first + 1;  // first round of the last finish attempt queued
     int64_t rounds = -1;
     bool first_group = true;
+    // A group that ends with the finish kernel gets its round flags (and the diagnostics) from the
+    // kernel's report in mapped memory; otherwise they are copied back.
+    bool reported = giant_ok && use_finish && first + 1 >= 2 && first + 1 <= last_round;
+    const uint32_t* hf = hflags.p;
     while (true) {
-      GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)dsu.nmod * 4, hipMemcpyDeviceToHost, st));
+      if (!reported) GZ_HIP(hipMemcpyAsync(hflags.p, mod.p, (size_t)dsu.nmod * 4, hipMemcpyDeviceToHost, st));
       if (first_group && speculate) (*speculate)();
       GZ_HIP(hipStreamSynchronize(st));
+      hf = reported ? rep.p : hflags.p;
       // rounds launched so far, or as far as a finish that took over went
-      const uint32_t upto = std::max(next - 1, hflags.p[dsu.nmod - 1]);
+      const uint32_t upto = std::max(next - 1, hf[dsu.nmod - 1]);
       for (uint32_t r = 0; r <= upto; ++r)
-        if (!hflags.p[r]) {
+        if (!hf[r]) {
           rounds = r;
           break;
         }
@@ -711,7 +717,7 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
       const uint32_t hi = std::min(next + 1, last_round);
       for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
-      if (giant_ok && use_finish) launch_finish(gb, hi + 1, last_round, slab.p, geo, dsu, rb, st);
+      reported = giant_ok && use_finish && launch_finish(gb, hi + 1, last_round, slab.p, geo, dsu, rb, st, rep.d);
       GZ_HIP(hipGetLastError());
       fin_r0 = hi + 1;
       next = hi + 1;
@@ -722,9 +728,15 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
       timed.back().items = (uint64_t)rounds + 1;
     }
     last_rounds = (size_t)rounds;
-    last_finish_start = hflags.p[dsu.nmod - 1] ? fin_r0 : 0;
-    GZ_HIP(hipMemcpyAsync(hround.p, r_stats.p, 128 * 4, hipMemcpyDeviceToHost, st));
-    GZ_HIP(hipEventRecord(hround_ev, st));
+    last_finish_start = hf[dsu.nmod - 1] ? fin_r0 : 0;
+    if (reported) {
+      std::memcpy(hround.p, rep.p + dsu.nmod, 128 * 4);
+      hround_on_event = false;
+    } else {
+      GZ_HIP(hipMemcpyAsync(hround.p, r_stats.p, 128 * 4, hipMemcpyDeviceToHost, st));
+      GZ_HIP(hipEventRecord(hround_ev, st));
+      hround_on_event = true;
+    }
     hround_pending = true;
     sf_pending = cfg.eager_dsu;
     dsu_valid = true;

@@ -295,7 +295,9 @@ class GzAlg {
   std::map<uint32_t, hipGraphExec_t> query_graphs;
   PinBuf<uint32_t> hround;  // per-round diagnostics of the last query (copied when it ends)
   hipEvent_t hround_ev = nullptr;  // that copy has landed
-  bool hround_pending = false;
+  bool hround_pending = false;      // hround holds the last query's diagnostics, not yet used
+  bool hround_on_event = false;     // ... once hround_ev has completed
+  MapBuf<uint32_t> rep;             // the finish kernel's report: round flags, then 128 diagnostic words
   uint32_t next_prefix = 0;  // K of the next query's prefix (0: none chosen yet)
   // The spanning-forest membership set that the eager DSU's deletion check reads is built from the
   // forest edges after a query, but only before the first eager batch that needs it.
