@@ -740,18 +740,6 @@ __global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, 
 // ================================================================================================
 // Borůvka (boruvka_emulation, src/cc_sketch_alg.cpp:464-513)
 // ================================================================================================
-__global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uint64_t sf_cap, uint64_t* sf_edge,
-                            uint32_t* flags, uint32_t* mod, uint32_t nmod) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (size_t i = t; i < n; i += stride) {
-    parent[i] = (uint32_t)i;
-    sf_edge[i] = kEmpty;
-  }
-  for (size_t i = t; i < sf_cap; i += stride) sf_hash[i] = kEmpty;
-  if (t < kNumFlags) flags[t] = 0;
-  for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
-}
 
 // Membership set of the spanning forest after a query (the eager DSU's deletion check): every
 // root linked by Borůvka holds its forest edge in sf_edge.
@@ -844,6 +832,23 @@ constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode 
 constexpr uint32_t kStWords = 8;
 constexpr uint32_t kCandidateDiv = 32;
 constexpr uint32_t kMaxStatRounds = 64;
+
+// cnt / st (a query's start, or nullptr): the root counts are cleared too, and the round state
+// words are set to their start values (no giant), so a query begins with one launch.
+__global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uint64_t sf_cap, uint64_t* sf_edge,
+                            uint32_t* flags, uint32_t* mod, uint32_t nmod, uint32_t* cnt, uint32_t* st) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (size_t i = t; i < n; i += stride) {
+    parent[i] = (uint32_t)i;
+    sf_edge[i] = kEmpty;
+    if (cnt) cnt[i] = 0;
+  }
+  for (size_t i = t; i < sf_cap; i += stride) sf_hash[i] = kEmpty;
+  if (t < kNumFlags) flags[t] = 0;
+  for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
+  if (st && t < kStWords) st[t] = (t == kStGiantRound || t == kStGiantFirst) ? kNoRoot : 0u;
+}
 
 __device__ inline uint32_t walk_count(const uint32_t* st, uint32_t round, uint32_t n) {
   return round >= 2 ? st[kStSeg + ((round - 1) & 1)] : n;
@@ -1814,10 +1819,11 @@ void launch_sf_build(const DsuDev& d, hipStream_t s) {
   hipLaunchKernelGGL(k_sf_build, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, s, d);
 }
 
-void launch_dsu_reset(const DsuDev& d, hipStream_t s) {
+void launch_dsu_reset(const DsuDev& d, hipStream_t s, const RoundBufs* rb) {
   const size_t work = std::max<size_t>(d.n, d.sf_mask + 1);
   hipLaunchKernelGGL(k_dsu_reset, dim3(grid_for(work, 256)), dim3(256), 0, s, d.parent, d.n, d.sf_hash,
-                     d.sf_mask + 1, d.sf_edge, d.flags, d.mod, d.nmod);
+                     d.sf_mask + 1, d.sf_edge, d.flags, d.mod, d.nmod, rb ? rb->cnt : nullptr,
+                     rb ? rb->totals : nullptr);
 }
 
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
@@ -1880,10 +1886,6 @@ bool launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned 
 }
 uint32_t finish_max() { return kFinishMax; }
 
-void launch_round_start(const RoundBufs& rb, hipStream_t s) {
-  launch_fill_u32(rb.totals, 0, kStWords, s);
-  launch_fill_u32(rb.totals + kStGiantRound, kNoRoot, 2, s);  // no giant round root, none elected
-}
 
 void launch_sum_all(int gbytes, const unsigned char* slab, const Geo& g, uint64_t* out_a, void* out_g, hipStream_t s) {
   launch_fill_u64(out_a, 0, g.nb, s);

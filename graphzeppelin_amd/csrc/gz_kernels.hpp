@@ -81,7 +81,9 @@ void launch_eager(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* m
                   hipStream_t s);
 
 // ---- Borůvka ----
-void launch_dsu_reset(const DsuDev& d, hipStream_t s);
+struct RoundBufs;
+// rb (a query's start): also clears rb->cnt and sets the round state words to their start values.
+void launch_dsu_reset(const DsuDev& d, hipStream_t s, const RoundBufs* rb = nullptr);
 struct RoundBufs {
   uint32_t* root;        // n      root of every walked vertex this round
   uint32_t* cnt;         // n      members per root (zero between rounds)
@@ -114,10 +116,8 @@ uint32_t finish_max();
 // the vertices may be summed from its complement.
 void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s);
-// Reset the round state of a query (before round 1).
 // Insert every recorded forest edge (sf_edge) into the forest membership set (after a query).
 void launch_sf_build(const DsuDev& d, hipStream_t s);
-void launch_round_start(const RoundBufs& rb, hipStream_t s);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
 // A query group's answers in place in mapped host memory: q[0..nq) and q[nq..2nq) are the pairs;
 // out gets the kNumFlags error flags, then one byte per pair (1: connected).

@@ -626,9 +626,7 @@ bool GzAlg::sketches_sum_to_zero() {
 // round last + 1, as one stream sequence (captured into a graph or launched directly).  Fills are
 // kernels, so the sequence holds no memset nodes.
 void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
-  launch_dsu_reset(dsu, st);
-  launch_fill_u32(r_cnt.p, 0, n, st);
-  launch_round_start(rb, st);
+  launch_dsu_reset(dsu, st, &rb);  // with the root counts and the round state words
   launch_sample_round0(gb, slab.p, geo, dsu, rb, st);
   for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
   if (giant_ok && use_finish) launch_finish(gb, last + 1, geo.samples - 1, slab.p, geo, dsu, rb, st, rep.d);
