@@ -1237,7 +1237,7 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
             const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
             XAcc xa, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
             const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats,
-            const unsigned char* __restrict__ det_cpy) {
+            const unsigned char* __restrict__ det_cpy, uint32_t small_walk) {
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
   GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
@@ -1251,7 +1251,11 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
     if (blockIdx.x == 0 && threadIdx.x == 0) samp[giant] = kSampZero;
     return;
   }
-  constexpr uint32_t span = kRedWaves * kWave;
+  // Entries per wave: 64, or 32 for a round of at most small_walk entries, whose few workgroups
+  // would otherwise each wait out a long chain of dependent gathers (the grid holds enough
+  // workgroups for either).
+  const uint32_t W = e1 <= small_walk ? kWave / 2 : kWave;
+  const uint32_t span = kRedWaves * W;
   const uint32_t blk = blockIdx.x;
   if (blk * span >= e1) return;  // uniform over the workgroup
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
@@ -1267,11 +1271,11 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
     for (uint32_t k = threadIdx.x; k < 2 * kWave; k += blockDim.x) s_tot[k] = 0;
   __syncthreads();
   // phase 1: every wave reduces its entries, leaving its boundary partials in LDS slots
-  const uint32_t p0 = b0 + wid * kWave;
+  const uint32_t p0 = b0 + wid * W;
   Slot hs{s_vals + (size_t)(2 * wid) * 2 * kWave, s_root + 2 * wid};
   Slot ts{s_vals + (size_t)(2 * wid + 1) * 2 * kWave, s_root + 2 * wid + 1};
   if (p0 < b1) {
-    const uint32_t len = b1 - p0 < kWave ? b1 - p0 : kWave;
+    const uint32_t len = b1 - p0 < W ? b1 - p0 : W;
     const uint32_t prev = p0 == 0 ? kNoRoot : root_at(p0 - 1);
     const uint32_t next = p0 + len >= e1 ? kNoRoot : root_at(p0 + len);
     uint64_t wa, wg;
@@ -1843,7 +1847,8 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
 // Host bound on each level's entry count (the kernels compute the exact counts on the device):
 // the larger of the chain without a supernode summed from its complement (level 2 = 2 * B1) and
 // with one (level 2 = 3 * B1); e1 <= n in both.
-uint32_t reduce_blocks(uint32_t n) { return std::max(1u, (n + kRedBlock - 1) / kRedBlock); }
+uint32_t reduce_blocks(uint32_t n) { return 2 * std::max(1u, (n + kRedBlock - 1) / kRedBlock); }
+constexpr uint32_t kSmallWalkDiv = 2;  // rounds regrouping at most n / kSmallWalkDiv entries use 32 per wave
 
 void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s) {
@@ -1864,10 +1869,10 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
   const XAcc xa{rb.xacc, rb.xcnt, rb.off, rb.cursor};
   if (gbytes == 4)
     hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(reduce_blocks(n)), dim3(kRedBlock), 0, s, st, members, rb.root,
-                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy);
+                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
   else
     hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(reduce_blocks(n)), dim3(kRedBlock), 0, s, st, members, rb.root,
-                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy);
+                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
                      act_list, prev, cur, rb.hint);
 }

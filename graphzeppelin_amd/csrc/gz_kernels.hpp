@@ -101,7 +101,7 @@ struct RoundBufs {
 // Writes rb.samp: a sample record per vertex.
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
                           hipStream_t s);
-// Workgroups of a round's reduce over n vertices (1024 member entries each).
+// Workgroups of a round's reduce over n vertices (enough for 512 member entries each).
 uint32_t reduce_blocks(uint32_t n);
 // Rounds r0, r0 + 1, ... up to last_round in one workgroup, when round r0 - 1 changed something, a
 // giant supernode exists and round r0 walks at most finish_max() vertices; otherwise a no-op.

@@ -175,7 +175,7 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
   // forest edge (8 each), spanning-forest hash set (16), deterministic-bucket copy, degree class
   // (1); the reduce's cross-workgroup accumulators (1 KiB + 4 B per 1024 vertices, and one more slot)
-  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 16 + 8 + gbytes + 1) + ((uint64_t)n / 1024 + 2) * 1028 + 8192;
+  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 16 + 8 + gbytes + 1) + ((uint64_t)n / 512 + 3) * 1028 + 8192;
   return slab + ingest + rounds;
 }
 
