@@ -1251,10 +1251,10 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
     if (blockIdx.x == 0 && threadIdx.x == 0) samp[giant] = kSampZero;
     return;
   }
-  // Entries per wave: 64, or 32 for a round of at most small_walk entries, whose few workgroups
-  // would otherwise each wait out a long chain of dependent gathers (the grid holds enough
-  // workgroups for either).
-  const uint32_t W = e1 <= small_walk ? kWave / 2 : kWave;
+  // Entries per wave: 64, or 32 for a round of at most small_walk entries (16 at most
+  // small_walk / 8), whose few workgroups would otherwise each wait out a long chain of dependent
+  // gathers (the grid holds enough workgroups for each choice).
+  const uint32_t W = e1 <= small_walk / 8 ? kWave / 4 : e1 <= small_walk ? kWave / 2 : kWave;
   const uint32_t span = kRedWaves * W;
   const uint32_t blk = blockIdx.x;
   if (blk * span >= e1) return;  // uniform over the workgroup
