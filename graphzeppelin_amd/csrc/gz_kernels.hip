@@ -759,9 +759,33 @@ __global__ void k_sf_build(DsuDev d) {
 template <class G>
 __global__ void __launch_bounds__(256)
 k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restrict__ samp,
-                uint32_t* __restrict__ flags, unsigned char* __restrict__ det_cpy, unsigned char* __restrict__ hint) {
+                uint32_t* __restrict__ flags, unsigned char* __restrict__ det_cpy, unsigned char* __restrict__ hint,
+                DsuDev d) {
   constexpr uint32_t V = 4, GL = kWave / V, GD = GL - 1;  // vertices per wave, lanes per vertex, depths
   constexpr uint32_t RB = 8 + sizeof(G);
+  // Round-0 hooking: every vertex is a root when round 0 begins, and only its own thread would link
+  // it under a higher-ranked vertex, so a sampled edge (v, u) whose other end u ranks above v
+  // (priority, then the smaller id) is merged here with plain stores -- v under u, parents always
+  // ranking higher, so the forest stays acyclic.  The other samples wait in samp for
+  // k_apply_samples, which links them with the CAS union.
+  bool hooked = false;
+  auto put = [&](uint32_t v, uint64_t rec) {
+    if (rec < kSampFail) {  // a GOOD sample: an edge index with both ends below n
+      const uint32_t x = (uint32_t)(rec >> 32), y = (uint32_t)rec;
+      const uint32_t u = x == v ? y : x;
+      if ((x == v || y == v) && u != v) {
+        const uint32_t pv = dsu_prio(v, d.prio_seed), pu = dsu_prio(u, d.prio_seed);
+        if (pu > pv || (pu == pv && u < v)) {
+          d.parent[v] = u;
+          d.sf_edge[v] = rec;  // the forest edge, in the slot of the root it linked
+          samp[v] = kSampZero;
+          hooked = true;
+          return;
+        }
+      }
+    }
+    samp[v] = rec;
+  };
   const uint32_t lane = lane_id(), i = lane % GL, base = lane - i;
   const uint32_t nd = g.D < GD ? g.D : GD;  // depths checked in the grouped pass
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
@@ -786,8 +810,8 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
     if (i == 0 && v < g.n) {
       hint[v] = (unsigned char)(gnz ? 32 - __builtin_clz(gnz) : 0);
       if (det_a == 0 && det_g == 0) samp[v] = kSampZero;
-      else if ((gmask >> GD) & 1) samp[v] = sample_record(kGood, det_a, g.n, flags);
-      else if (colm) samp[v] = sample_record(kGood, col_a, g.n, flags);
+      else if ((gmask >> GD) & 1) put(v, sample_record(kGood, det_a, g.n, flags));
+      else if (colm) put(v, sample_record(kGood, col_a, g.n, flags));
       else if (g.D <= GD) samp[v] = kSampFail;
       else undecided = true;
     }
@@ -795,15 +819,16 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
     if (i == GD && v < g.n) st_rec<G>(det_cpy + (size_t)v * RB, a, gm);
     for (uint64_t und = __ballot(undecided); und; und &= und - 1) {  // uniform over the wave
       const uint32_t u = v0 + (uint32_t)__builtin_ctzll(und) / GL;
-      const uint32_t d = GD + lane;  // D - GD <= 64 lanes for every supported n
+      const uint32_t dep = GD + lane;  // D - GD <= 64 lanes for every supported n
       uint64_t b = 0;
       G bg = 0;
-      if (d < g.D) ld_rec<G>(slab + (size_t)u * g.row + (size_t)d * RB, &b, &bg);
-      const uint64_t m2 = __ballot(d < g.D && (G)xxh3_mix(xxh3_prep(b), g.chk_bf) == bg);
+      if (dep < g.D) ld_rec<G>(slab + (size_t)u * g.row + (size_t)dep * RB, &b, &bg);
+      const uint64_t m2 = __ballot(dep < g.D && (G)xxh3_mix(xxh3_prep(b), g.chk_bf) == bg);
       const uint64_t ia = m2 ? rd_lane_u64(b, __builtin_ctzll(m2)) : 0;
-      if (lane == 0) samp[u] = m2 ? sample_record(kGood, ia, g.n, flags) : kSampFail;
+      if (lane == 0) put(u, m2 ? sample_record(kGood, ia, g.n, flags) : kSampFail);
     }
   }
+  if (__ballot(hooked) && lane_id() == 0) d.mod[0] = 1;  // round 0 merged something
 }
 
 // The merges of a round (sample_supernode, src/cc_sketch_alg.cpp:137-166), one thread per
@@ -1836,10 +1861,10 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
   const uint32_t grid = grid_for((size_t)(g.n + 3) / 4 * kWave, 256, 16384);
   if (gbytes == 4)
     hipLaunchKernelGGL(k_sample_round0<uint32_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy,
-                       rb.hint);
+                       rb.hint, d);
   else
     hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy,
-                       rb.hint);
+                       rb.hint, d);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr, 0u,
                      nullptr, nullptr, d.mod, rb.hint);
 }
