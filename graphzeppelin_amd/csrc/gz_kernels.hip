@@ -704,10 +704,17 @@ inline uint32_t grid_for(size_t work, uint32_t block, uint32_t cap = 8192) {
 // src != pairs: the batch is also copied from src into pairs (the staging buffer) on the way, so
 // ingesting device pairs needs no separate copy launch; the copy happens whether or not the DSU
 // is still valid.
+// build_sf: the first batch after a query also fills the forest membership set from the query's
+// forest edges (sf_edge) -- lock-free inserts like this batch's own, so the two need no order.
 __global__ void k_eager_union(const uint2* __restrict__ src, uint2* __restrict__ pairs, size_t count,
-                              uint8_t* __restrict__ merged, DsuDev d) {
+                              uint8_t* __restrict__ merged, DsuDev d, uint32_t build_sf) {
   const bool live = !ld_acq(d.flags + kFlagEagerInvalid);
   if (!live && src == pairs) return;
+  if (build_sf && live)
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += gridDim.x * blockDim.x) {
+      const uint64_t e = d.sf_edge[v];
+      if (e != kEmpty) sf_insert(d, e);
+    }
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
     const uint2 p = src[i];
     if (src != pairs) pairs[i] = p;
@@ -741,14 +748,6 @@ __global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, 
 // Borůvka (boruvka_emulation, src/cc_sketch_alg.cpp:464-513)
 // ================================================================================================
 
-// Membership set of the spanning forest after a query (the eager DSU's deletion check): every
-// root linked by Borůvka holds its forest edge in sf_edge.
-__global__ void k_sf_build(DsuDev d) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += gridDim.x * blockDim.x) {
-    const uint64_t e = d.sf_edge[v];
-    if (e != kEmpty) sf_insert(d, e);
-  }
-}
 
 // Round 0 (run_round_zero, 223-244): every vertex samples column 0 of its own sketch
 // (Sketch::sample, src/sketch.cpp:94-116).  A wave takes four vertices in groups of 16 lanes: lane
@@ -890,8 +889,8 @@ __device__ inline bool apply_sample(const DsuDev& d, uint64_t s, const unsigned 
   if (child == kNoLink) return false;
   // A merge records its edge in the spanning forest (cc_sketch_alg.cpp:152-162), in the slot of
   // the root it linked (every root is linked at most once between resets, so slots never
-  // collide); the membership set the eager DSU uses is built once after the last round
-  // (k_sf_build), off the merge's critical path.
+  // collide); the membership set the eager DSU uses is filled from these slots by the first eager
+  // batch after the query (k_eager_union), off the merge's critical path.
   d.sf_edge[child] = a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a;
   return true;
 }
@@ -1835,18 +1834,16 @@ void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile
 size_t part_hist_len() { return (size_t)kPartBins * kPartMaxBlocks + 1; }
 
 void launch_eager(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d,
-                  hipStream_t s) {
+                  hipStream_t s, bool build_sf) {
   if (!count) return;
   uint2* p2 = reinterpret_cast<uint2*>(pairs);
   const uint32_t grid = grid_for(count, 256);
-  hipLaunchKernelGGL(k_eager_union, dim3(grid), dim3(256), 0, s, reinterpret_cast<const uint2*>(src), p2, count, merged,
-                     d);
+  const uint32_t ugrid = build_sf ? std::max(grid, grid_for(d.n, 256, 4096)) : grid;
+  hipLaunchKernelGGL(k_eager_union, dim3(ugrid), dim3(256), 0, s, reinterpret_cast<const uint2*>(src), p2, count,
+                     merged, d, build_sf ? 1u : 0u);
   hipLaunchKernelGGL(k_eager_sf_check, dim3(grid), dim3(256), 0, s, p2, count, merged, d);
 }
 
-void launch_sf_build(const DsuDev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_sf_build, dim3(grid_for(d.n, 256, 4096)), dim3(256), 0, s, d);
-}
 
 void launch_dsu_reset(const DsuDev& d, hipStream_t s, const RoundBufs* rb) {
   const size_t work = std::max<size_t>(d.n, d.sf_mask + 1);

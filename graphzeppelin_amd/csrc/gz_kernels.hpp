@@ -77,8 +77,10 @@ void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile
 
 // ---- eager DSU (batch form of pre_insert, src/cc_sketch_alg.cpp:79-104) ----
 // src != pairs: the batch is copied from src into pairs by the same launch.
+// build_sf: the same launch first fills the forest membership set from d.sf_edge (the first
+// batch after a query).
 void launch_eager(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* merged, const DsuDev& d,
-                  hipStream_t s);
+                  hipStream_t s, bool build_sf = false);
 
 // ---- Borůvka ----
 struct RoundBufs;
@@ -116,8 +118,6 @@ uint32_t finish_max();
 // the vertices may be summed from its complement.
 void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
                   const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s);
-// Insert every recorded forest edge (sf_edge) into the forest membership set (after a query).
-void launch_sf_build(const DsuDev& d, hipStream_t s);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
 // A query group's answers in place in mapped host memory: q[0..nq) and q[nq..2nq) are the pairs;
 // out gets the kNumFlags error flags, then one byte per pair (1: connected).

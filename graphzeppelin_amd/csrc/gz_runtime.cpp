@@ -749,11 +749,9 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
 }
 
 void GzAlg::eager_batch(const uint32_t* src, uint32_t* pairs, size_t count, uint8_t* merged) {
-  if (sf_pending) {
-    launch_sf_build(dsu, st);
-    sf_pending = false;
-  }
-  launch_eager(src, pairs, count, merged, dsu, st);
+  if (!count) return;
+  launch_eager(src, pairs, count, merged, dsu, st, sf_pending);  // builds the forest set when pending
+  sf_pending = false;
 }
 
 // connected_components, src/cc_sketch_alg.cpp:515-557 + ConnectedComponents, return_types.cpp:5-19
