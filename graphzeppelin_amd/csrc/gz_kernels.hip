@@ -677,12 +677,75 @@ k_part_group(const uint2* __restrict__ rec, uint32_t n, uint32_t shift, uint32_t
   }
 }
 
-void scan_counts(const uint32_t* cnt, uint32_t n, uint32_t tile, uint64_t* partials, uint32_t* off, uint32_t* toff,
-                 uint32_t* cursor, uint32_t* totals, hipStream_t s) {
+// The apply step of the scan when there are at most kScanBlock partials: every block adds up the
+// partials before its own (one per thread), so the separate block-sums launch is not needed, and
+// the last block writes the totals.  tiles != nullptr: each block also writes its vertices' tiles
+// (what k_make_tiles does), with the profiling bytes when acc != nullptr.
+__global__ void __launch_bounds__(kScanBlock)
+k_scan_apply_direct(const uint32_t* __restrict__ cnt, uint32_t n, uint32_t tile, const uint64_t* __restrict__ partials,
+                    uint32_t nparts, uint32_t* __restrict__ off, uint32_t* __restrict__ toff,
+                    uint32_t* __restrict__ cursor, uint32_t* __restrict__ totals, Tile* __restrict__ tiles,
+                    uint64_t* __restrict__ acc, uint64_t S, uint64_t per_update) {
+  uint64_t prefix;
+  block_excl_scan_u64(threadIdx.x < blockIdx.x ? partials[threadIdx.x] : 0, &prefix);
+  const uint32_t base = blockIdx.x * kScanSpan + threadIdx.x * kScanItems;
+  uint32_t c[kScanItems];
+  uint64_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t i = base + k;
+    c[k] = i < n ? cnt[i] : 0;
+    s += pack_ct(c[k], tile);
+  }
+  uint64_t tot;
+  uint64_t run = block_excl_scan_u64(s, &tot) + prefix;
+  uint64_t bytes = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t i = base + k;
+    if (i < n) {
+      off[i] = (uint32_t)run;
+      if (toff) toff[i] = (uint32_t)(run >> 32);
+      if (cursor) cursor[i] = (uint32_t)run;
+      if (tiles && c[k]) {
+        const uint32_t nt = (c[k] + tile - 1) / tile, o = (uint32_t)run, t0 = (uint32_t)(run >> 32);
+        for (uint32_t j = 0; j < nt; ++j) {
+          const uint32_t b = j * tile;
+          tiles[t0 + j] = Tile{i, o + b, (c[k] - b) < tile ? (c[k] - b) : tile, nt > 1 ? 1u : 0u};
+        }
+        const uint64_t touched = (uint64_t)c[k] * per_update;
+        bytes += 4ull * c[k] + 2 * (touched < S ? touched : S);
+      }
+    }
+    run += pack_ct(c[k], tile);
+  }
+  if (blockIdx.x == nparts - 1 && threadIdx.x == 0) {
+    const uint64_t all = prefix + tot;
+    totals[0] = (uint32_t)all;
+    totals[1] = (uint32_t)(all >> 32);
+    off[n] = (uint32_t)all;
+    if (toff) toff[n] = (uint32_t)(all >> 32);
+  }
+  if (acc) {  // profiling: the block's algorithmic bytes
+    for (int o = 32; o > 0; o >>= 1) bytes += shfl_xor_u64(bytes, o);
+    if (lane_id() == 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(acc + blockIdx.x % kProfSlots), (unsigned long long)bytes);
+  }
+}
+
+// Returns whether the tile list was written too (only when tiles != nullptr and the one-pass apply
+// was used).
+bool scan_counts(const uint32_t* cnt, uint32_t n, uint32_t tile, uint64_t* partials, uint32_t* off, uint32_t* toff,
+                 uint32_t* cursor, uint32_t* totals, hipStream_t s, Tile* tiles = nullptr,
+                 const FlushBytes* fb = nullptr) {
   const uint32_t nparts = (n + kScanSpan - 1) / kScanSpan;
   hipLaunchKernelGGL(k_scan_partials, dim3(nparts), dim3(kScanBlock), 0, s, cnt, n, tile, partials);
+  if (nparts <= kScanBlock) {
+    hipLaunchKernelGGL(k_scan_apply_direct, dim3(nparts), dim3(kScanBlock), 0, s, cnt, n, tile, partials, nparts, off,
+                       toff, cursor, totals, tiles, fb ? fb->acc : nullptr, fb ? fb->S : 0, fb ? fb->per_update : 0);
+    return tiles != nullptr;
+  }
   hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanBlock), 0, s, partials, nparts, n, off, toff, totals);
   hipLaunchKernelGGL(k_scan_apply, dim3(nparts), dim3(kScanBlock), 0, s, cnt, n, tile, partials, off, toff, cursor);
+  return false;
 }
 
 inline uint32_t grid_for(size_t work, uint32_t block, uint32_t cap = 8192) {
@@ -1820,7 +1883,7 @@ void launch_group(const uint32_t* pairs, size_t count, uint32_t n, uint32_t tile
                          b.rec);
     hipLaunchKernelGGL(k_part_group, dim3(bins), dim3(kPartBlock), 0, s, b.rec, n, shift, bins, nblk, b.hoff, b.cnt,
                        b.dst_arr);
-    scan_counts(b.cnt, n, tile, b.partials, b.off, b.toff, nullptr, b.totals, s);
+    if (scan_counts(b.cnt, n, tile, b.partials, b.off, b.toff, nullptr, b.totals, s, b.tiles, fb)) return;
   } else {
     launch_fill_u32(b.cnt, 0, n, s);
     if (count) hipLaunchKernelGGL(k_count_pairs, dim3(grid_for(count, 256)), dim3(256), 0, s, p2, count, n, b.cnt, b.bad);
