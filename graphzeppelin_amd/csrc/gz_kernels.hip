@@ -1,13 +1,15 @@
 // gz_kernels.hip -- hand-written gfx950 kernels for GraphZeppelin's streaming-CC hot path.
 //
-// Data layout in HBM (one "slab" per algorithm instance):
-//   alpha plane  u64 [n][stride]   bucket b of vertex v at alpha[v*stride + b]
-//   gamma plane  G   [n][stride]   G = u32 (12-byte reference bucket) or u64 (16-byte bucket)
-//   bucket b = c*D + d for column c, depth d; the deterministic bucket is b = C*D (last), exactly
-//   the reference's bucket order (src/sketch.cpp:69-83), so packing the two planes back into
-//   {alpha, gamma} records reproduces the reference's byte image.
+// Data layout in HBM (one "slab" per algorithm instance): row v holds vertex v's sketch as packed
+// bucket records {u64 alpha; G gamma} (G = u32 for the 12-byte reference bucket, u64 for the
+// 16-byte one).  Column c starts on a 128-B line at byte c * cpitch and holds depth d at
+// c * cpitch + d * sizeof(record); the deterministic bucket follows column 0's D records.  Kernels
+// address buckets by the reference's index b = c*D + d (deterministic bucket last, b = C*D,
+// src/sketch.cpp:69-83), so packing a row reproduces the reference's byte image.
 // Everything is integer work; nothing here is a dense contraction, so no MFMA is used.
 #include <hip/hip_runtime.h>
+
+#include <stdexcept>
 
 #include "gz_kernels.hpp"
 
@@ -932,7 +934,9 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
     if (cnt) cnt[i] = 0;
   }
   for (size_t i = t; i < sf_cap; i += stride) sf_hash[i] = kEmpty;
-  if (t < kNumFlags) flags[t] = 0;
+  // The bad-input flag is left alone: pairs grouped by the flush just before a query set it, and the
+  // query's answer reports it (it is cleared where it is reported).
+  if (t < kNumFlags && t != kFlagBadInput) flags[t] = 0;
   for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
   if (st && t < kStWords) st[t] = (t == kStGiantRound || t == kStGiantFirst) ? kNoRoot : 0u;
 }
@@ -1158,11 +1162,13 @@ struct Slot {
   uint32_t* root;
 };
 
-// Cross-workgroup accumulators: (gridDim.x + 1) slots of 128 u64 and as many arrival counters, all
-// zero between rounds.
+// Cross-workgroup accumulators: nslots slots of 128 u64 and as many arrival counters, all zero
+// between rounds.  Slots 0 .. gridDim.x - 1 belong to the workgroups, slot nslots - 1 to the giant;
+// the launcher checks gridDim.x + 1 <= nslots.
 struct XAcc {
   uint64_t* vals;
   uint32_t* cnt;
+  uint32_t nslots;
   const uint32_t* seg_off;  // segment start per root
   const uint32_t* seg_end;  // segment end per root (the scatter cursor after k_bv_scatter)
 };
@@ -1405,7 +1411,7 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
   const bool has_t = tr != kNoRoot && tr != hr;  // a supernode covering the whole range is counted once
   const uint32_t fh = has_h ? s_bnd[0] / span : 0, lh = has_h ? (s_bnd[1] - 1) / span : 0;
   const uint32_t lt = has_t ? (s_bnd[2] - 1) / span : 0;
-  const uint32_t gslot = gridDim.x;
+  const uint32_t gslot = xa.nslots - 1;
   if (has_h) xacc_add(xa.vals + (size_t)fh * 2 * kWave, s_out[lane], s_out[kWave + lane]);
   if (has_t) xacc_add(xa.vals + (size_t)blk * 2 * kWave, s_out[2 * kWave + lane], s_out[3 * kWave + lane]);
   if (giant_tot) xacc_add(xa.vals + (size_t)gslot * 2 * kWave, s_tot[lane], s_tot[kWave + lane]);
@@ -1950,13 +1956,15 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
                      giant_ok ? 1u : 0u, prev);
   hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, st, round,
                      act_list, prev);
-  // the accumulators hold reduce_blocks(n) + 1 slots; the grid size names the giant's slot
-  const XAcc xa{rb.xacc, rb.xcnt, rb.off, rb.cursor};
+  // one accumulator slot per workgroup and one for the giant
+  const uint32_t rgrid = reduce_blocks(n);
+  if (rgrid + 1 > rb.xslots) throw std::logic_error("reduce grid exceeds its cross-workgroup accumulator slots");
+  const XAcc xa{rb.xacc, rb.xcnt, rb.xslots, rb.off, rb.cursor};
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(reduce_blocks(n)), dim3(kRedBlock), 0, s, st, members, rb.root,
+    hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.root,
                        slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
   else
-    hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(reduce_blocks(n)), dim3(kRedBlock), 0, s, st, members, rb.root,
+    hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.root,
                        slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
                      act_list, prev, cur, rb.hint);

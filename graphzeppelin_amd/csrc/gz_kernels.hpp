@@ -93,8 +93,9 @@ struct RoundBufs {
   uint32_t* cursor;      // n      scatter cursor per root (the segment end after the scatter)
   uint32_t* members[2];  // n each vertices grouped by root, by round parity (also the next active list)
   uint32_t* totals;      // 8      round state words (entry counts, giant roots, walk mode)
-  uint64_t* xacc;        // (reduce_blocks(n) + 1) * 128: cross-workgroup supernode sums, zero between rounds
-  uint32_t* xcnt;        // reduce_blocks(n) + 1 arrival counters, zero between rounds
+  uint64_t* xacc;        // xslots * 128: cross-workgroup supernode sums, zero between rounds
+  uint32_t* xcnt;        // xslots arrival counters, zero between rounds
+  uint32_t xslots;       // >= reduce_blocks(n) + 1 (one per reduce workgroup, one for the giant)
   uint64_t* samp;        // n      sample record per supernode root
   uint32_t* rstats;      // 2 * 64 per round: vertices regrouped, giant root (diagnostics)
   unsigned char* det_cpy;  // n records: every vertex's deterministic bucket, copied in round 0

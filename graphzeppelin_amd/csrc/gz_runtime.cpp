@@ -222,6 +222,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   dsu.nmod = geo.samples + 2;
   dsu.prio_seed = (uint32_t)(seed * 0x9E3779B97F4A7C15ULL >> 32);
   dsu.n = n;
+  launch_fill_u32(flags.p, 0, kNumFlags, st);  // the reset below keeps the bad-input flag
   launch_dsu_reset(dsu, st);
 
   // Borůvka round buffers
@@ -239,7 +240,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   launch_fill_u64(r_xacc.p, 0, xslots * 128, st);
   launch_fill_u32(r_xcnt.p, 0, xslots, st);
   rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
-                 r_xacc.p, r_xcnt.p, r_samp.p, r_stats.p, r_det.p, r_hint.p};
+                 r_xacc.p, r_xcnt.p, (uint32_t)xslots, r_samp.p, r_stats.p, r_det.p, r_hint.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
 
   // device ingest

@@ -154,6 +154,7 @@ class Server {
  public:
   struct Tenant {
     std::unique_ptr<GzAlg> alg;
+    std::unique_ptr<gz_alg> view;  // borrowed handle given out by gz_server_tenant_alg
     uint64_t footprint = 0;
   };
   uint64_t budget;
@@ -182,18 +183,21 @@ class Server {
       reserved -= fp;
       throw;
     }
+    std::unique_ptr<gz_alg> view(new gz_alg{a.get()});
     std::lock_guard<std::mutex> lk(mtx);
     const int id = next_id++;
-    tenants[id] = Tenant{std::move(a), fp};
+    tenants[id] = Tenant{std::move(a), std::move(view), fp};
     return id;
   }
   void release(int id) {
     std::unique_ptr<GzAlg> a;
+    std::unique_ptr<gz_alg> view;
     {
       std::lock_guard<std::mutex> lk(mtx);
       auto it = tenants.find(id);
       if (it == tenants.end()) throw GzError(GZ_ERR_INVALID_ARG, "unknown tenant");
       a = std::move(it->second.alg);
+      view = std::move(it->second.view);
       reserved -= it->second.footprint;
       tenants.erase(it);
     }
@@ -203,6 +207,12 @@ class Server {
     auto it = tenants.find(id);
     if (it == tenants.end()) throw GzError(GZ_ERR_INVALID_ARG, "unknown tenant");
     return it->second.alg.get();
+  }
+  gz_alg* view(int id) {
+    std::lock_guard<std::mutex> lk(mtx);
+    auto it = tenants.find(id);
+    if (it == tenants.end()) throw GzError(GZ_ERR_INVALID_ARG, "unknown tenant");
+    return it->second.view.get();
   }
 
   // Mixed replay; see gz_server_run_trace.
@@ -468,10 +478,9 @@ int gz_server_release(gz_server* s, int tenant) {
 int gz_server_tenant_alg(gz_server* s, int tenant, gz_alg** out) {
   return gz::guarded_call([&] {
     if (!s || !out) throw gz::GzError(GZ_ERR_INVALID_ARG, "null argument");
-    // borrowed view: the server owns the instance; gz_alg_destroy must not be called on it
-    static thread_local gz_alg view;
-    view.a = s->s->get(tenant);
-    *out = &view;
+    // borrowed handle: the server owns the instance and the handle, which stays valid until the
+    // tenant is released; gz_alg_destroy must not be called on it
+    *out = s->s->view(tenant);
   });
 }
 uint64_t gz_server_reserved_bytes(const gz_server* s) {

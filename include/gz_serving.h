@@ -68,6 +68,9 @@ int gz_server_destroy(gz_server* s);
  * footprint (gz_alg_footprint_bytes) does not fit in the unreserved budget. */
 int gz_server_admit(gz_server* s, uint32_t num_vertices, uint64_t seed, const gz_alg_config* cfg, int* tenant);
 int gz_server_release(gz_server* s, int tenant);
+/* A borrowed handle to an admitted tenant's instance (owned by the server; never pass it to
+ * gz_alg_destroy).  It stays valid until the tenant is released or the server destroyed; every call
+ * for one tenant returns the same handle. */
 int gz_server_tenant_alg(gz_server* s, int tenant, gz_alg** out);
 uint64_t gz_server_reserved_bytes(const gz_server* s);
 uint64_t gz_server_budget_bytes(const gz_server* s);

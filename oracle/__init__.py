@@ -57,6 +57,8 @@ _SIGS = {
     "or_alg_spanning_forest": (_sz, [_vp, _vp, _sz]),
     "or_alg_dump": (None, [_vp, _u32, _u32, _vp]),
     "or_exact_cc": (_u32, [_u32, _vp, _sz, _vp]),
+    "or_alg_write_binary": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "or_alg_construct_from_serialized": (_vp, [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "or_baseline_trace": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _vp, _sz, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double), _vp, _vp]),
 }
@@ -240,6 +242,23 @@ class CCAlg:
         out = np.empty(count * self.sketch_bytes(), dtype=np.uint8)
         lib().or_alg_dump(self._h, v0, count, _p(out))
         return out
+
+    def write_binary(self, path: str):
+        """CCSketchAlg::write_binary (src/cc_sketch_alg.cpp:614-623)."""
+        if lib().or_alg_write_binary(self._h, path.encode()):
+            raise OSError(f"cannot write {path}")
+
+    @classmethod
+    def construct_from_serialized_data(cls, path: str, batch_factor=1.0, gamma_bytes=4, eager=True):
+        """CCSketchAlg::construct_from_serialized_data (src/cc_sketch_alg.cpp:30-64)."""
+        h = lib().or_alg_construct_from_serialized(path.encode(), batch_factor, gamma_bytes, 1 if eager else 0)
+        if not h:
+            raise OSError(f"cannot read {path}")
+        obj = cls.__new__(cls)
+        obj._h = h
+        with open(path, "rb") as f:
+            obj.n = int.from_bytes(f.read(12)[8:12], "little")
+        return obj
 
     def baseline_trace(self, pairs, qpos, kinds, qa, qb, nthreads):
         """Reference-style CPU run over a request trace; returns (wall_s, latencies, answers)."""

@@ -669,6 +669,44 @@ void or_alg_dump(void* p, uint32_t v0, uint32_t count, void* out) {
     for (uint32_t i = 0; i < count; ++i) std::memcpy((char*)out + (size_t)i * sb, A.sk[v0 + i]->b.data(), sb);
   });
 }
+// CCSketchAlg::write_binary, src/cc_sketch_alg.cpp:614-623: seed (size_t), num_vertices
+// (node_id_t), sketches_factor (double), then every vertex's Sketch::serialize image.  Returns 0, or
+// 1 when the file cannot be written.
+int or_alg_write_binary(void* p, const char* path) {
+  auto* h = (OrAlg*)p;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 1;
+  bool ok = true;
+  DISPATCH_ALG(h, {
+    const uint64_t seed = A.seed;
+    const uint32_t nv = A.n;
+    const double sf = A.sketches_factor;
+    ok = std::fwrite(&seed, 8, 1, f) == 1 && std::fwrite(&nv, 4, 1, f) == 1 && std::fwrite(&sf, 8, 1, f) == 1;
+    for (uint32_t v = 0; v < nv && ok; ++v) ok = std::fwrite(A.sk[v]->b.data(), A.sk[v]->bytes(), 1, f) == 1;
+  });
+  return (std::fclose(f) == 0 && ok) ? 0 : 1;
+}
+// CCSketchAlg::construct_from_serialized_data + the stream constructor, src/cc_sketch_alg.cpp:30-64:
+// the header's sketches_factor replaces the configured one, each sketch is read back, and the DSU
+// starts invalid (no cached answer).  Returns nullptr when the file is missing or truncated.
+void* or_alg_construct_from_serialized(const char* path, double bf, int gbytes, int eager) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return nullptr;
+  uint64_t seed = 0;
+  uint32_t nv = 0;
+  double sf = 1;
+  bool ok = std::fread(&seed, 8, 1, f) == 1 && std::fread(&nv, 4, 1, f) == 1 && std::fread(&sf, 8, 1, f) == 1;
+  if (!ok || nv < 2) { std::fclose(f); return nullptr; }
+  auto* h = (OrAlg*)or_alg_create(nv, seed, sf, bf, gbytes, eager);
+  DISPATCH_ALG(h, {
+    for (uint32_t v = 0; v < nv && ok; ++v) ok = std::fread(A.sk[v]->b.data(), A.sk[v]->bytes(), 1, f) == 1;
+    A.dsu_valid = false;
+    A.shared_dsu_valid = false;
+  });
+  std::fclose(f);
+  if (!ok) { or_alg_destroy(h); return nullptr; }
+  return h;
+}
 uint32_t or_exact_cc(uint32_t n, const uint32_t* pairs, size_t count, uint32_t* labels) {
   return exact_cc(n, pairs, count, labels);
 }

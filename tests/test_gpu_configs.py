@@ -12,6 +12,7 @@ import os
 import numpy as np
 import pytest
 
+import exact_cc
 import graphzeppelin_amd as gz
 import oracle
 from graphzeppelin_amd import streams
@@ -68,8 +69,8 @@ def test_rmat_2e17_queries_every_10m():
         lo, hi = q * step, (q + 1) * step
         alg.ingest_pairs_device(pairs[lo:hi].data_ptr(), hi - lo)
         cc = alg.connected_components()
-        keys = gen.prefix_final_keys(pairs, hi)
-        c, lab = gen.exact_components_torch(n, keys)
+        keys = exact_cc.prefix_final_keys(pairs, hi)
+        c, lab = exact_cc.exact_components_torch(n, keys)
         say(f"rmat: query {q} at {hi}: components {cc.size()} exact {c} rounds {alg.last_query_rounds}")
         assert cc.size() == c
         assert np.array_equal(cc.canonical_labels(), lab.cpu().numpy().astype(np.uint32))

@@ -228,21 +228,37 @@ def test_no_eager_dsu_always_runs_boruvka():
     assert alg.has_cached_query()
 
 
-def test_write_binary_round_trip(tmp_path):
-    # write_binary / construct_from_serialized_data (src/cc_sketch_alg.cpp:614-623, 30-64)
-    pairs, _ = streams.erdos_renyi_churn_stream(300, 0.05, 9)
-    alg = gz.CCSketchAlg(300, 1234, gz.CCAlgConfiguration(sketches_factor=1.5))
+@pytest.mark.parametrize("gb", [4, 8])
+def test_write_binary_equals_oracle_checkpoint(tmp_path, gb):
+    # write_binary / construct_from_serialized_data (src/cc_sketch_alg.cpp:614-623, 30-64) at a
+    # non-default sketches_factor: the device's file equals the oracle's file byte for byte, and
+    # both reloaded instances give the same round count and partition as the exact DSU
+    n, seed, sf = 300, 1234, 2.5  # 28 columns instead of the default 18
+    pairs, _ = streams.erdos_renyi_churn_stream(n, 0.05, 9)
+    alg = gz.CCSketchAlg(n, seed, gz.CCAlgConfiguration(sketches_factor=sf, gamma_bytes=gb))
     alg.ingest_pairs(pairs)
-    path = str(tmp_path / "alg.bin")
+    ref = oracle.CCAlg(n, seed, sketches_factor=sf, gamma_bytes=gb)
+    ref.ingest_pairs(pairs)
+    path, rpath = str(tmp_path / "alg.bin"), str(tmp_path / "ref.bin")
     alg.write_binary(path)
-    raw = open(path, "rb").read()
-    assert int.from_bytes(raw[:8], "little") == 1234 and int.from_bytes(raw[8:12], "little") == 300
-    assert np.frombuffer(raw[12:20], dtype="<f8")[0] == 1.5
-    assert np.array_equal(np.frombuffer(raw[20:], dtype=np.uint8), alg.dump_sketches())
-    back = gz.CCSketchAlg.construct_from_serialized_data(path)
-    assert not back.has_cached_query()
-    assert np.array_equal(back.dump_sketches(), alg.dump_sketches())
-    assert back.connected_components().size() == alg.connected_components().size()
+    ref.write_binary(rpath)
+    raw, rraw = open(path, "rb").read(), open(rpath, "rb").read()
+    assert int.from_bytes(raw[:8], "little") == seed and int.from_bytes(raw[8:12], "little") == n
+    assert np.frombuffer(raw[12:20], dtype="<f8")[0] == sf
+    assert alg.sketch_bytes() == (28 * 17 + 1) * (8 + gb) == ref.sketch_bytes()
+    assert len(raw) == 20 + n * alg.sketch_bytes()
+    assert raw == rraw
+    back = gz.CCSketchAlg.construct_from_serialized_data(path, gz.CCAlgConfiguration(gamma_bytes=gb))
+    rback = oracle.CCAlg.construct_from_serialized_data(rpath, gamma_bytes=gb)
+    assert not back.has_cached_query() and not rback.has_cached_query()
+    assert np.array_equal(back.dump_sketches(), rback.dump())
+    cc = back.connected_components()
+    rpar, rc = rback.connected_components()
+    c, lab = oracle.exact_cc(n, pairs)
+    assert cc.size() == rc == c
+    assert np.array_equal(cc.canonical_labels(), oracle.canonical(rpar))
+    assert np.array_equal(cc.canonical_labels(), lab)
+    assert back.last_query_rounds == rback.last_query_rounds
 
 
 def test_apply_raw_buckets_update_parity():

@@ -71,6 +71,36 @@ def test_serve_trace_answers_exact(eager):
     assert gz.digest_of_image(img, ref.sketch_bytes() // 12, 4) == ref.digest()
 
 
+@pytest.mark.parametrize("eager", [True, False])
+def test_out_of_range_device_pair_is_reported_by_a_query(eager):
+    """A device pair naming a vertex >= n is skipped and reported as INVALID_ARG by the next query,
+    also when that query runs Borůvka (the query's DSU reset keeps the flag); the instance then
+    answers from the valid pairs alone."""
+    import torch
+
+    say(f"out-of-range device pair eager={eager}")
+    n = 512
+    pairs, _ = streams.erdos_renyi_churn_stream(n, 0.02, 5)
+    bad = pairs.copy()
+    bad[len(bad) // 2] = (3, n)  # one pair names vertex n
+    dev = torch.from_numpy(bad.astype(np.int32)).cuda()
+    alg = gz.CCSketchAlg(n, 4, gz.CCAlgConfiguration(eager_dsu=eager))
+    trace = np.array([(gz.REQ_UPDATES, 0, 0, bad.shape[0], 0, 0, 0.0), (gz.REQ_POINT, 0, 0, 0, 1, 2, 0.0)],
+                     dtype=gz.REQUEST_DTYPE)
+    with pytest.raises(gz.GZError) as ei:
+        alg.serve_trace(dev.data_ptr(), True, trace)
+    assert ei.value.code == 1, ei.value  # GZ_ERR_INVALID_ARG
+    # the flag was cleared where it was reported; the sketches hold every valid pair
+    keep = np.ones(bad.shape[0], dtype=bool)
+    keep[len(bad) // 2] = False
+    c, lab = oracle.exact_cc(n, bad[keep])
+    cc = alg.connected_components()
+    assert cc.size() == c and np.array_equal(cc.canonical_labels(), lab)
+    ref = oracle.CCAlg(n, 4)
+    ref.ingest_pairs(bad[keep])
+    assert np.array_equal(alg.dump_sketches(), ref.dump())
+
+
 def test_plain_and_batched_forms_agree():
     say("plain vs batched n=2048")
     n = 2048

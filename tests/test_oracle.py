@@ -263,9 +263,13 @@ def test_batching_independence(oracle_mod):
 def test_config1_stream_shape():
     pairs, types = streams.erdos_renyi_churn_stream(1024, 0.1, 1)
     streams.check_type_consistent(pairs, types)
-    m = int((types == 0).sum() - (types == 1).sum())
-    n_ins_first = pairs.shape[0] - int((types == 1).sum()) * 1.1
-    assert 51000 < m + int((types == 1).sum()) * 0.9 < 54000 or n_ins_first > 0
+    # G(1024, 0.1) has 52 378 expected edges (sd ~ 220); 20% of them are deleted and 10% of those
+    # reinserted, so the stream holds E first inserts, 0.2 E deletes and 0.02 E reinserts
+    ins, dels = int((types == 0).sum()), int((types == 1).sum())
+    e = ins - dels // 10  # first inserts, up to rounding of the reinsert count
+    assert 51000 < e < 54000
+    assert abs(dels - 0.2 * e) <= 2
+    assert abs((ins - e) - 0.1 * dels) <= 2
     assert 60000 < pairs.shape[0] < 68000
 
 
@@ -278,3 +282,28 @@ def test_cpu_baseline_trace_matches_exact(oracle_mod):
     c2, _ = oracle_mod.exact_cc(512, pairs)
     assert list(ans) == [c1, c2]
     assert wall > 0 and np.all(lat > 0)
+
+
+def test_checkpoint_file_layout_and_reload(oracle_mod, tmp_path):
+    # write_binary (src/cc_sketch_alg.cpp:614-623): size_t seed, node_id_t n, double
+    # sketches_factor, then every vertex's serialize image in vertex order; the stream constructor
+    # (30-64) reads it back with the header's sketches_factor and no cached answer
+    pairs, _ = streams.erdos_renyi_churn_stream(200, 0.05, 4)
+    a = oracle_mod.CCAlg(200, 77, sketches_factor=2.5)
+    a.ingest_pairs(pairs)
+    path = str(tmp_path / "ck.bin")
+    a.write_binary(path)
+    raw = open(path, "rb").read()
+    seed, n, sf = struct.unpack("<QId", raw[:20])
+    assert (seed, n, sf) == (77, 200, 2.5)
+    assert np.array_equal(np.frombuffer(raw[20:], dtype=np.uint8), a.dump())
+    # sketches_factor 2.5 changes the geometry: ceil(2.5 log2(200) / (1 - log2 1.2)) = 26 columns
+    # instead of the minimum 18 (D = 16 at n = 200), so 26 * 16 + 1 buckets of 12 bytes
+    assert a.sketch_bytes() == (26 * 16 + 1) * 12
+    assert oracle_mod.CCAlg(200, 77).sketch_bytes() == (18 * 16 + 1) * 12
+    b = oracle_mod.CCAlg.construct_from_serialized_data(path)
+    assert not b.has_cached_query()
+    assert np.array_equal(b.dump(), a.dump())
+    par, c = b.connected_components()
+    ce, lab = oracle_mod.exact_cc(200, pairs)
+    assert c == ce and np.array_equal(oracle_mod.canonical(par), lab)
