@@ -35,39 +35,6 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_trace(n: int, total: int, seed: int, per_request: int, bmin: int = 1000, bmax: int = 1_000_000):
-    import graphzeppelin_amd as gz
-
-    rng = np.random.default_rng(seed)
-    rows = []
-    off = 0
-    while off < total:
-        b = int(np.exp(rng.uniform(np.log(bmin), np.log(bmax))))
-        b = min(b, total - off)
-        rows.append((gz.REQ_UPDATES, 0, off, b, 0, 0, 0.0))
-        off += b
-        for _ in range(rng.poisson(b / per_request)):
-            if rng.random() < 0.01:
-                rows.append((gz.REQ_CC, 0, 0, 0, 0, 0, 0.0))
-            else:
-                u, v = rng.integers(0, n, 2)
-                rows.append((gz.REQ_POINT, 0, 0, 0, int(u), int(v), 0.0))
-    return np.array(rows, dtype=gz.REQUEST_DTYPE)
-
-
-def prefix_trace(trace: np.ndarray, upto: int) -> np.ndarray:
-    """Requests that lie before update `upto` (the last batch clipped)."""
-    out = []
-    for r in trace:
-        if r["kind"] == 0:
-            if r["offset"] >= upto:
-                break
-            r = r.copy()
-            r["count"] = min(int(r["count"]), upto - int(r["offset"]))
-        out.append(r)
-    return np.array(out, dtype=trace.dtype)
-
-
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_update_tiles_chunglu_n262144_traffic.md")
 VALU_FILE = os.path.join(REPO, "profiles", "k_update_tiles_chunglu_n262144_valu.md")
 # wave-level VALU issue peak: 256 CUs x 4 SIMDs x one wave64 instruction per 4 cycles x 2.4 GHz
@@ -159,7 +126,7 @@ def main() -> int:
     del types
     torch.cuda.synchronize()
     log(f"stream ready in {time.time() - t0:.1f}s ({pairs.numel() * 4 / 2**30:.2f} GiB in HBM)")
-    trace = make_trace(n, N, args.seed + 7, args.per_request)
+    trace = gen.serving_trace(n, N, args.seed + 7, args.per_request)
     nq = int((trace["kind"] != 0).sum())
     log(f"trace: {trace.size} requests, {nq} queries")
     cfg = gz.CCAlgConfiguration(gamma_bytes=args.gamma_bytes, tile=args.tile, flush_threshold=args.flush_threshold)
@@ -275,7 +242,7 @@ def main() -> int:
     if not args.no_plain:
         P = min(args.plain_sample, N)
         host = pairs[:P].cpu().numpy().astype(np.uint32)
-        ptrace = prefix_trace(trace, P)
+        ptrace = gen.trace_prefix(trace, P)
         plain = gz.CCSketchAlg(n, args.sketch_seed, gz.CCAlgConfiguration(
             gamma_bytes=args.gamma_bytes, batched=False, tile=args.tile))
         log(f"plain form on the first {P} updates")
@@ -302,7 +269,7 @@ def main() -> int:
 
         P = min(args.cpu_sample, N)
         host = pairs[:P].cpu().numpy().astype(np.uint32)
-        ctrace = prefix_trace(trace, P)
+        ctrace = gen.trace_prefix(trace, P)
         q = ctrace[ctrace["kind"] != 0]
         qpos = []
         upto = 0

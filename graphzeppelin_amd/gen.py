@@ -165,3 +165,42 @@ def chung_lu_stream(n: int, n_updates: int, del_frac: float, seed: int, device="
 
 def keys_to_pairs(keys: torch.Tensor) -> torch.Tensor:
     return torch.stack([(keys >> 32).to(torch.int32), (keys & 0xFFFFFFFF).to(torch.int32)], dim=1).contiguous()
+
+
+def serving_trace(n: int, total: int, seed: int, per_request: int, bmin: int = 1000, bmax: int = 1_000_000,
+                  cc_frac: float = 0.01, tenant: int = 0):
+    """Request trace of the serving loop over a stream of `total` updates: update batches of
+    log-uniform size in [bmin, bmax], each followed by Poisson(batch / per_request) query requests,
+    a `cc_frac` share of them connected_components() and the rest u-v point queries on uniform
+    vertex pairs.  Rows are gz_request records (graphzeppelin_amd.REQUEST_DTYPE), arrival times 0."""
+    from ._lib import REQ_CC, REQ_POINT, REQ_UPDATES, REQUEST_DTYPE
+
+    rng = np.random.default_rng(seed)
+    rows = []
+    off = 0
+    while off < total:
+        b = int(np.exp(rng.uniform(np.log(bmin), np.log(bmax))))
+        b = min(b, total - off)
+        rows.append((REQ_UPDATES, tenant, off, b, 0, 0, 0.0))
+        off += b
+        for _ in range(rng.poisson(b / per_request)):
+            if rng.random() < cc_frac:
+                rows.append((REQ_CC, tenant, 0, 0, 0, 0, 0.0))
+            else:
+                u, v = rng.integers(0, n, 2)
+                rows.append((REQ_POINT, tenant, 0, 0, int(u), int(v), 0.0))
+    return np.array(rows, dtype=REQUEST_DTYPE)
+
+
+def trace_prefix(trace: np.ndarray, upto: int) -> np.ndarray:
+    """The requests of `trace` that come before update `upto` (the last update batch clipped)."""
+    out = []
+    for r in trace:
+        if r["kind"] == 0:
+            if r["offset"] >= upto:
+                break
+            r = r.copy()
+            r["count"] = min(int(r["count"]), upto - int(r["offset"]))
+        out.append(r)
+    return np.array(out, dtype=trace.dtype)
+

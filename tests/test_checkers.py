@@ -1,0 +1,39 @@
+"""The exact-components checker (tests/exact_cc.py) against the oracle's exact DSU on the CPU, before
+the GPU tests trust it: typed R-MAT and Chung-Lu streams, components at many prefixes."""
+import numpy as np
+import pytest
+
+import exact_cc
+
+
+@pytest.mark.parametrize("model", ["rmat", "chung_lu"])
+def test_lifetime_components_equal_exact_dsu(oracle_mod, model):
+    import torch
+
+    from graphzeppelin_amd import gen
+
+    if model == "rmat":
+        n = 1 << 10
+        pairs, types, _ = gen.rmat_stream(10, 20_000, 0.3, 5, device="cpu")
+    else:
+        n = 3000
+        pairs, types, _ = gen.chung_lu_stream(n, 8_000, 0.3, 6, device="cpu", mean_deg=3.0)
+    host = pairs.numpy().astype(np.uint32)
+    life = exact_cc.EdgeLifetimes(pairs)
+    assert int((life.dele < life.N).sum()) == int(types.sum())  # every delete pairs with its insert
+    for p in [0, 1, 17, 500, 4_000, pairs.shape[0] // 2, pairs.shape[0] - 1, pairs.shape[0]]:
+        ncc, lab = life.components(n, p)
+        c, want = oracle_mod.exact_cc(n, host[:p])
+        assert ncc == c, p
+        assert np.array_equal(lab.numpy().astype(np.uint32), want), p
+
+
+def test_components_of_edges_on_a_path_and_isolated_vertices():
+    import torch
+
+    # a path 9-8-...-1 (labels must travel the whole path) plus isolated 0 and 10
+    u = torch.arange(1, 9, dtype=torch.int64)
+    v = u + 1
+    ncc, lab = exact_cc.components_of_edges(11, v.flip(0), u.flip(0))
+    assert ncc == 3
+    assert lab.tolist() == [0] + [1] * 9 + [10]
