@@ -124,6 +124,7 @@ _SIGS = {
     "gz_sketch_destroy": (ctypes.c_int, [_vp]),
     "gz_sketch_update": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "gz_sketch_update_batch": (ctypes.c_int, [_vp, _vp, _sz]),
+    "gz_sketch_update_batch_device_noalloc": (ctypes.c_int, [_vp, _vp, _sz]),
     "gz_sketch_sample": (ctypes.c_int, [_vp, _u64p, ctypes.POINTER(ctypes.c_int)]),
     "gz_sketch_merge": (ctypes.c_int, [_vp, _vp]),
     "gz_sketch_range_merge": (ctypes.c_int, [_vp, _vp, _sz, _sz]),
@@ -291,6 +292,11 @@ class Sketch:
     def update_batch(self, idx):
         a = np.ascontiguousarray(np.asarray(idx, dtype=np.uint64))
         _check(lib().gz_sketch_update_batch(self._h, _ptr(a), a.size))
+
+    def update_batch_device_noalloc(self, idx_device_ptr: int, count: int):
+        """update_batch over `count` uint64 indices in device memory, allocating nothing (at most
+        2^26 indices per call; calls that overlap on one GPU race)."""
+        _check(lib().gz_sketch_update_batch_device_noalloc(self._h, ctypes.c_void_p(idx_device_ptr), count))
 
     def sample(self):
         i = ctypes.c_uint64()

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgz_hip.so")
-SOURCES = ["gz_kernels.hip", "gz_runtime.cpp", "gz_serving.cpp"]
+SOURCES = ["gz_kernels.hip", "gz_runtime.cpp", "gz_serving.cpp", "gz_noalloc.hip"]
 ARCH = os.environ.get("GZ_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 ALU ops.

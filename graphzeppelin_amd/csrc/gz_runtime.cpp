@@ -68,6 +68,7 @@ GzSketch::GzSketch(uint64_t vector_len, uint64_t seed, size_t samples, size_t co
   require_device();
   geo = make_geo(1, vector_len, seed, samples, cols, gb);
   if (update_tiles_lds_bytes(geo, gb) > 64 * 1024) throw GzError(GZ_ERR_INVALID_ARG, "sketch too large for one workgroup's LDS");
+  GZ_HIP(hipGetDevice(&dev));
   GZ_HIP(hipStreamCreate(&st));
   slab.alloc(geo.row);
   out.alloc(4);
@@ -923,6 +924,14 @@ int gz::guarded_call(const std::function<void()>& f) {
     return GZ_ERR_DEVICE;
   }
 }
+
+int gz::status_of(const std::exception& e) {
+  g_last_error = e.what();
+  if (const auto* ge = dynamic_cast<const GzError*>(&e)) return ge->code;
+  if (dynamic_cast<const std::bad_alloc*>(&e)) return GZ_ERR_OOM;
+  return GZ_ERR_DEVICE;
+}
+void gz::clear_last_error() { g_last_error.clear(); }
 
 template <class F>
 static int guarded(F&& f) {

@@ -121,6 +121,7 @@ class GzSketch {
  public:
   Geo geo{};
   int gb;
+  int dev = 0;  // the device the sketch's buffers and stream live on
   size_t sample_idx = 0;
   hipStream_t st = nullptr;
   DevBuf<uint8_t> slab;  // packed bucket records (see Geo::row)
@@ -310,6 +311,10 @@ class GzAlg {
 
 // Runs f, converting exceptions to a gz_status and the thread's last-error message.
 int guarded_call(const std::function<void()>& f);
+// The same conversion for entry points that catch their own exceptions: the status of e (set as
+// the thread's last-error message), and clear_last_error() after a successful call.
+int status_of(const std::exception& e);
+void clear_last_error();
 
 }  // namespace gz
 

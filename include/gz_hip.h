@@ -58,6 +58,13 @@ int gz_sketch_destroy(gz_sketch* s);
 /* Sketch::update, src/sketch.cpp:69-83 (one index, or a batch applied in one launch) */
 int gz_sketch_update(gz_sketch* s, uint64_t update_idx);
 int gz_sketch_update_batch(gz_sketch* s, const uint64_t* idx, size_t count);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): Sketch::update over `count`
+ * indices already in device memory, with the kernel and the result of gz_sketch_update_batch, and
+ * no allocation or staging.  The kernel's tile descriptors live in a static device array of 1 MiB
+ * per GPU, so count is at most 2^26 (GZ_ERR_INVALID_ARG above that).  Every call on a GPU shares
+ * that array: calls that overlap on one GPU race.  Returns when the update is done. */
+int gz_sketch_update_batch_device_noalloc(gz_sketch* s, const uint64_t* idx_device, size_t count);
 /* Sketch::sample, src/sketch.cpp:94-116 -> *result is a gz_sample_result */
 int gz_sketch_sample(gz_sketch* s, uint64_t* idx, int* result);
 /* Sketch::merge / range_merge / merge_raw_bucket_buffer, src/sketch.cpp:149-186 */
