@@ -187,7 +187,7 @@ _SIGS = {
     "gz_server_reserved_bytes": (ctypes.c_uint64, [_vp]),
     "gz_server_budget_bytes": (ctypes.c_uint64, [_vp]),
     "gz_server_run_trace": (ctypes.c_int, [_vp, _vp, _sz, _vp, _vp, _vp, _sz, ctypes.POINTER(_AlgConfig), ctypes.c_int,
-                                           _vp, _vp, _vp, ctypes.POINTER(ServeStats)]),
+                                           _vp, _vp, _vp, _vp, ctypes.POINTER(ServeStats)]),
 }
 
 
@@ -673,7 +673,10 @@ class Server:
     def release(self, tenant: int):
         _check(lib().gz_server_release(self._h, tenant))
 
-    def run_trace(self, reqs: np.ndarray, tenant_ptrs, tenant_n, tenant_seed, config=None, release_when_done=True):
+    def run_trace(self, reqs: np.ndarray, tenant_ptrs, tenant_n, tenant_seed, config=None, release_when_done=True,
+                  want_parents=False):
+        """Replays a mixed trace; returns (latencies, answers, digests, stats), plus each tenant's
+        final parent array when want_parents."""
         r = np.ascontiguousarray(reqs, dtype=REQUEST_DTYPE)
         T = len(tenant_n)
         ptrs = (ctypes.c_void_p * T)(*[ctypes.c_void_p(int(p)) for p in tenant_ptrs])
@@ -682,9 +685,15 @@ class Server:
         lat = np.zeros(r.size, dtype=np.float64)
         ans = np.zeros(r.size, dtype=np.uint32)
         dig = np.zeros(T, dtype=np.uint64)
+        par = [np.zeros(int(n), dtype=np.uint32) for n in tn] if want_parents else None
+        par_ptrs = (ctypes.c_void_p * T)(*[_ptr(p) for p in par]) if want_parents else None
         st = ServeStats()
         c = (config or CCAlgConfiguration()).to_c()
         _check(lib().gz_server_run_trace(self._h, _ptr(r), r.size, ctypes.cast(ptrs, ctypes.c_void_p), _ptr(tn),
                                          _ptr(ts), T, ctypes.byref(c), 1 if release_when_done else 0, _ptr(lat),
-                                         _ptr(ans), _ptr(dig), ctypes.byref(st)))
+                                         _ptr(ans), _ptr(dig),
+                                         ctypes.cast(par_ptrs, ctypes.c_void_p) if want_parents else None,
+                                         ctypes.byref(st)))
+        if want_parents:
+            return lat, ans, dig, st.as_dict(), par
         return lat, ans, dig, st.as_dict()

@@ -218,12 +218,12 @@ class Server {
   // Mixed replay; see gz_server_run_trace.
   void run_trace(const gz_request* reqs, size_t nreq, const uint32_t* const* tpairs, const uint32_t* tn,
                  const uint64_t* tseed, size_t T, const gz_alg_config& cfg, bool release_done, double* lat,
-                 uint32_t* answers, uint64_t* digests, gz_serve_stats& total);
+                 uint32_t* answers, uint64_t* digests, uint32_t* const* parents, gz_serve_stats& total);
 };
 
 void Server::run_trace(const gz_request* reqs, size_t nreq, const uint32_t* const* tpairs, const uint32_t* tn,
                        const uint64_t* tseed, size_t T, const gz_alg_config& cfg, bool release_done, double* lat,
-                       uint32_t* answers, uint64_t* digests, gz_serve_stats& total) {
+                       uint32_t* answers, uint64_t* digests, uint32_t* const* parents, gz_serve_stats& total) {
   struct TState {
     std::deque<size_t> todo;  // request indices in trace order
     int id = -1;              // server tenant id once admitted
@@ -366,6 +366,7 @@ void Server::run_trace(const gz_request* reqs, size_t nreq, const uint32_t* cons
         if (last) {
           A->flush_all();
           if (digests) digests[pick] = A->digest();
+          if (parents && parents[pick]) A->connected_components(parents[pick]);
         }
         lk.lock();
         s.busy = false;
@@ -492,14 +493,15 @@ uint64_t gz_server_budget_bytes(const gz_server* s) { return s ? s->s->budget : 
 
 int gz_server_run_trace(gz_server* s, const gz_request* reqs, size_t nreq, const uint32_t* const* tpairs,
                         const uint32_t* tn, const uint64_t* tseed, size_t T, const gz_alg_config* cfg,
-                        int release_done, double* lat, uint32_t* answers, uint64_t* digests, gz_serve_stats* stats) {
+                        int release_done, double* lat, uint32_t* answers, uint64_t* digests,
+                        uint32_t* const* parents, gz_serve_stats* stats) {
   return gz::guarded_call([&] {
     if (!s || (!reqs && nreq) || !tpairs || !tn || !tseed) throw gz::GzError(GZ_ERR_INVALID_ARG, "null argument");
     gz_alg_config c;
     gz_alg_config_default(&c);
     if (cfg) c = *cfg;
     gz_serve_stats st{};
-    s->s->run_trace(reqs, nreq, tpairs, tn, tseed, T, c, release_done != 0, lat, answers, digests, st);
+    s->s->run_trace(reqs, nreq, tpairs, tn, tseed, T, c, release_done != 0, lat, answers, digests, parents, st);
     if (stats) *stats = st;
   });
 }

@@ -30,8 +30,11 @@ def _gen(seed: int, device) -> torch.Generator:
     return g
 
 
-def rmat_candidates(scale: int, count: int, a: float, b: float, c: float, g: torch.Generator, device) -> torch.Tensor:
-    """Edge keys (min << 32 | max, int64) of `count` R-MAT draws; self loops are encoded as -1."""
+def rmat_candidates(scale: int, count: int, a: float, b: float, c: float, g: torch.Generator, device,
+                    n: int | None = None) -> torch.Tensor:
+    """Edge keys (min << 32 | max, int64) of `count` R-MAT draws on 2^scale vertices; self loops, and
+    draws with an endpoint >= n when n is given (R-MAT restricted to the first n vertices), are
+    encoded as -1."""
     out = torch.empty(count, dtype=torch.int64, device=device)
     for s in range(0, count, _CHUNK):
         m = min(_CHUNK, count - s)
@@ -45,7 +48,10 @@ def rmat_candidates(scale: int, count: int, a: float, b: float, c: float, g: tor
             dst = dst * 2 + bit_d
         lo, hi = torch.minimum(src, dst), torch.maximum(src, dst)
         key = (lo << 32) | hi
-        key[src == dst] = -1
+        bad = src == dst
+        if n is not None:
+            bad |= hi >= n
+        key[bad] = -1
         out[s:s + m] = key
     return out
 
@@ -142,13 +148,15 @@ def _inserts(cand_fn, n_ins: int, g, device) -> torch.Tensor:
 
 
 def rmat_stream(scale: int, n_updates: int, del_frac: float, seed: int, device="cuda", a=0.57, b=0.19, c=0.19,
-                want_final: bool = False):
-    """R-MAT(a, b, c, d = 1-a-b-c) stream on 2^scale vertices with del_frac of the updates deleting
-    present edges."""
+                want_final: bool = False, n: int | None = None):
+    """R-MAT(a, b, c, d = 1-a-b-c) stream on 2^scale vertices (or its restriction to the first n
+    of them, n <= 2^scale) with del_frac of the updates deleting present edges."""
+    if n is not None and not (0 < n <= (1 << scale)):
+        raise ValueError("n must lie in (0, 2^scale]")
     g = _gen(seed, device)
     n_del = int(round(n_updates * del_frac))
     n_ins = n_updates - n_del
-    keys = _inserts(lambda cnt: rmat_candidates(scale, cnt, a, b, c, g, device), n_ins, g, device)
+    keys = _inserts(lambda cnt: rmat_candidates(scale, cnt, a, b, c, g, device, n), n_ins, g, device)
     return typed_stream(keys, n_del, g, want_final)
 
 
@@ -203,4 +211,58 @@ def trace_prefix(trace: np.ndarray, upto: int) -> np.ndarray:
             r["count"] = min(int(r["count"]), upto - int(r["offset"]))
         out.append(r)
     return np.array(out, dtype=trace.dtype)
+
+
+def multi_tenant_workload(num_tenants: int, seed: int, log2_lo: float = 12.0, log2_hi: float = 20.0,
+                          updates_per_nlogn: float = 8.0, del_frac: float = 0.1, device="cuda"):
+    """Tenants of the multi-tenant serving configuration: n log-uniform in [2^log2_lo, 2^log2_hi],
+    each an R-MAT(0.57, 0.19, 0.19, 0.05) stream of updates_per_nlogn * n * log2(n) updates with
+    del_frac deletes of present edges, drawn in chunks from torch's counter-based (Philox) generator
+    seeded per tenant.  Returns [(n, pairs int32 (N, 2) on `device`)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(num_tenants):
+        n = int(round(2.0 ** rng.uniform(log2_lo, log2_hi)))
+        scale = max(1, math.ceil(math.log2(n)))
+        m = int(round(updates_per_nlogn * n * math.log2(n)))
+        pairs, _, _ = rmat_stream(scale, m, del_frac, seed * 1000 + t, device=device, n=n)
+        out.append((n, pairs))
+    return out
+
+
+def multi_tenant_trace(tenants, seed: int, rate_updates_per_s: float | None, cc_per_tenant: float = 4.0,
+                       point_frac: float = 0.5, bmin: int = 1000, bmax: int = 1_000_000):
+    """Mixed request trace over tenants [(n, pairs)]: each tenant's stream arrives as update batches
+    of log-uniform size in [bmin, bmax] in stream order; after a batch come Poisson-many query
+    requests (on average cc_per_tenant connected_components() requests per tenant, at uniformly
+    random stream positions, plus as many u-v point queries times point_frac / (1 - point_frac)),
+    and every tenant ends with one connected_components() request.  Arrival times: every tenant's
+    updates arrive at the same steady rate, so that all tenants together offer rate_updates_per_s
+    (None: everything arrives at time 0); a query arrives with the batch before it.  Requests are
+    ordered by arrival time."""
+    from ._lib import REQ_CC, REQ_POINT, REQ_UPDATES, REQUEST_DTYPE
+
+    rng = np.random.default_rng(seed)
+    rows = []
+    T = len(tenants)
+    for t, (n, pairs) in enumerate(tenants):
+        total = int(pairs.shape[0])
+        per_tenant_rate = None if rate_updates_per_s is None else rate_updates_per_s / T
+        qrate = (cc_per_tenant / (1.0 - point_frac)) / total  # queries per update
+        off = 0
+        while off < total:
+            b = min(int(np.exp(rng.uniform(np.log(bmin), np.log(bmax)))), total - off)
+            arr = 0.0 if per_tenant_rate is None else (off + b) / per_tenant_rate
+            rows.append((REQ_UPDATES, t, off, b, 0, 0, arr))
+            off += b
+            for _ in range(rng.poisson(b * qrate)):
+                if rng.random() < point_frac:
+                    u, v = rng.integers(0, n, 2)
+                    rows.append((REQ_POINT, t, 0, 0, int(u), int(v), arr))
+                else:
+                    rows.append((REQ_CC, t, 0, 0, 0, 0, arr))
+        rows.append((REQ_CC, t, 0, 0, 0, 0, 0.0 if per_tenant_rate is None else total / per_tenant_rate))
+    arr = np.array(rows, dtype=REQUEST_DTYPE)
+    order = np.argsort(arr["arrival_s"], kind="stable")  # stable: a tenant's requests keep their order
+    return arr[order]
 

@@ -82,11 +82,12 @@ uint64_t gz_server_budget_bytes(const gz_server* s);
  * parameters for admission (tenant ids index these arrays).  Requests of one tenant are applied in
  * trace order, none before its arrival time; latency = completion - arrival.  Different tenants'
  * requests overlap on the device.  tenant_digest[t] (may be NULL) receives gz_alg_digest of tenant t
- * after its last request. */
+ * after its last request; tenant_parents (may be NULL, as may each tenant_parents[t]) receives the
+ * tenant's connected_components() parent array (tenant_n[t] DSU roots) after its last request. */
 int gz_server_run_trace(gz_server* s, const gz_request* reqs, size_t nreq, const uint32_t* const* tenant_pairs,
                         const uint32_t* tenant_n, const uint64_t* tenant_seed, size_t num_tenants,
                         const gz_alg_config* cfg, int release_when_done, double* latency_s, uint32_t* answers,
-                        uint64_t* tenant_digest, gz_serve_stats* stats);
+                        uint64_t* tenant_digest, uint32_t* const* tenant_parents, gz_serve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -37,3 +37,25 @@ def test_components_of_edges_on_a_path_and_isolated_vertices():
     ncc, lab = exact_cc.components_of_edges(11, v.flip(0), u.flip(0))
     assert ncc == 3
     assert lab.tolist() == [0] + [1] * 9 + [10]
+
+
+def test_multi_tenant_trace_shape():
+    from graphzeppelin_amd import gen
+    import graphzeppelin_amd as gz
+
+    tenants = gen.multi_tenant_workload(5, 3, log2_lo=8, log2_hi=10, device="cpu")
+    for n, p in tenants:
+        assert 256 <= n <= 1024 and int(p.max()) < n
+        assert abs(p.shape[0] - 8 * n * np.log2(n)) <= 1
+    trace = gen.multi_tenant_trace(tenants, 4, 1e6, bmin=10, bmax=1000)
+    assert np.all(np.diff(trace["arrival_s"]) >= 0)
+    for t, (n, p) in enumerate(tenants):
+        sub = trace[trace["tenant"] == t]
+        upd = sub[sub["kind"] == gz.REQ_UPDATES]
+        # the stream arrives once, in order, at the tenant's steady rate
+        assert upd["offset"][0] == 0 and np.array_equal(upd["offset"][1:], np.cumsum(upd["count"])[:-1])
+        assert int(upd["count"].sum()) == p.shape[0]
+        assert np.allclose(upd["arrival_s"], (upd["offset"] + upd["count"]) / (1e6 / 5))
+        assert sub[-1]["kind"] == gz.REQ_CC  # every stream ends with a connected_components() request
+        pts = sub[sub["kind"] == gz.REQ_POINT]
+        assert np.all(pts["u"] < n) and np.all(pts["v"] < n)
