@@ -84,6 +84,22 @@ def build_cpp_tests(verbose: bool = False) -> str:
     return out
 
 
+def build_harness(verbose: bool = False) -> str:
+    """Compile the end-to-end stream harness (csrc/gz_process_stream.cpp, the counterpart of the
+    reference's tools/process_stream.cpp) against libgz_hip.so, next to the library."""
+    src = os.path.join(CSRC, "gz_process_stream.cpp")
+    out = os.path.join(HERE, "gz_process_stream")
+    cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-I", os.path.join(REPO, "include"), src, "-o", out,
+           "-L", HERE, "-lgz_hip", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("stream harness build failed\n" + res.stdout + res.stderr)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_cpp_tests(verbose=True)
+    build_harness(verbose=True)

@@ -68,8 +68,10 @@ struct Geo {
   uint32_t cps;      // cols_per_sample
   uint32_t samples;  // num_samples
   uint32_t nb;       // buckets per sketch = C*D + 1 (deterministic bucket last)
-  uint32_t row;      // bytes between consecutive sketches in the slab (row_bytes)
-  uint32_t cpitch;   // bytes between consecutive columns of a row (col_pitch_bytes)
+  uint32_t row;      // bytes between consecutive sketches in the slab (RowLayout::row)
+  uint32_t low;      // depths of a column kept in the paired low region (RowLayout::low)
+  uint32_t ovf;      // byte offset of the overflow region in a row (RowLayout::ovf)
+  uint32_t opitch;   // bytes between consecutive columns in the overflow region (RowLayout::opitch)
   uint32_t Dp;       // LDS column pitch (D rounded up to 1 mod 16 to spread LDS banks)
   uint64_t seed;     // sketch seed (checksum seed)
   uint64_t chk_bf;   // bitflip of the checksum seed
@@ -77,12 +79,24 @@ struct Geo {
 
 inline uint32_t lds_pitch(uint32_t D) { return D + ((17u - (D % 16u)) % 16u); }
 
-// Row of one vertex sketch in the slab: column c's D records (depth d at byte c*cpitch + d*rb),
-// each column starting on a 128-B cache line; the deterministic bucket follows column 0's records
-// (byte D*rb), in that column's padding.  rb = bytes of one bucket record.
-inline uint32_t col_pitch_bytes(uint32_t D, uint32_t rb) { return ((D + 1) * rb + 127u) & ~127u; }
-inline uint64_t row_bytes(uint32_t C, uint32_t D, uint32_t rb) {
-  return ((uint64_t)C * col_pitch_bytes(D, rb) + 255u) & ~(uint64_t)255u;
+// Row of one vertex sketch in the slab.  An update touches one record in every column, at depth d
+// with probability 2^-(d+1), and the deterministic bucket, so the low depths are what every flush
+// reads and writes.  The low region pairs columns on 128-B lines: depths d < low of column c sit at
+// byte (c / 2) * 128 + (c % 2) * 64 + d * rb (low = the records that fit in 64 B: 5 of 12 B, 4 of
+// 16 B), and the deterministic bucket takes slot C of the same pairing (the free half of the last
+// line when C is odd).  Depths d >= low of column c follow in the overflow region at byte
+// ovf + c * opitch + (d - low) * rb, each column's run starting on a 128-B line, so a Borůvka round
+// reads column r as one low half-line and ceil((D - low) * rb / 128) overflow lines.
+struct RowLayout {
+  uint32_t low, ovf, opitch, row;
+};
+inline RowLayout row_layout(uint32_t C, uint32_t D, uint32_t rb) {
+  RowLayout L{};
+  L.low = std::min(D, 64u / rb);
+  L.ovf = (C + 2) / 2 * 128u;  // ceil((C + 1) / 2) lines: C columns and the deterministic bucket
+  L.opitch = D > L.low ? ((D - L.low) * rb + 127u) & ~127u : 0u;
+  L.row = (L.ovf + C * L.opitch + 255u) & ~255u;
+  return L;
 }
 
 // Per-tile work descriptor of the update kernel: `count` destinations of `vertex` starting at

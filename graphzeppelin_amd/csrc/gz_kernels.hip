@@ -2,10 +2,10 @@
 //
 // Data layout in HBM (one "slab" per algorithm instance): row v holds vertex v's sketch as packed
 // bucket records {u64 alpha; G gamma} (G = u32 for the 12-byte reference bucket, u64 for the
-// 16-byte one).  Column c starts on a 128-B line at byte c * cpitch and holds depth d at
-// c * cpitch + d * sizeof(record); the deterministic bucket follows column 0's D records.  Kernels
-// address buckets by the reference's index b = c*D + d (deterministic bucket last, b = C*D,
-// src/sketch.cpp:69-83), so packing a row reproduces the reference's byte image.
+// 16-byte one).  The low depths of two columns share a 128-B line (with the deterministic bucket in
+// the pairing's last slot) and deeper depths follow per column in an overflow region (RowLayout in
+// gz_common.hpp).  Kernels address buckets by the reference's index b = c*D + d (deterministic
+// bucket last, b = C*D, src/sketch.cpp:69-83), so packing a row reproduces the reference's image.
 // Everything is integer work; nothing here is a dense contraction, so no MFMA is used.
 #include <hip/hip_runtime.h>
 
@@ -72,23 +72,24 @@ __device__ inline void st_rlx(T* p, T v) {
 
 // ---- the slab: packed bucket records -----------------------------------------------------------
 // Every bucket is the record {u64 alpha; G gamma} -- the reference's packed Bucket, 12 or 16 bytes.
-// Row v of the slab holds sketch v.  Column c starts on a 128-B cache line (byte c * g.cpitch) and
-// holds depth d at c * g.cpitch + d * (8 + sizeof(G)), so the low depths of a column, which most
-// updates touch, share one line with their checksums.  The deterministic bucket follows column 0's
-// records, in that column's padding: the deterministic bucket and column 0 -- all that round 0
-// reads -- are one contiguous run, with the deterministic bucket in lane D of a wave that reads
-// it.  A 12-byte record is only 4-byte aligned, so it is moved as 32-bit words, and a 64-bit XOR
-// atomic on it becomes two 32-bit ones (XOR acts bit by bit, so the result is the same).
-// rec_col_off: byte of (column c, depth d) within a row; rec_off: byte of the reference's bucket
-// index b (c * D + d, the deterministic bucket last) of sketch v.
+// Row v of the slab holds sketch v (layout in gz_common.hpp, RowLayout): depths below g.low of two
+// columns share a 128-B line, with the deterministic bucket in the pairing's slot C; deeper depths
+// of each column follow in the overflow region, each column's run on its own lines.  A 12-byte
+// record is only 4-byte aligned, so it is moved as 32-bit words, and a 64-bit XOR atomic on it
+// becomes two 32-bit ones (XOR acts bit by bit, so the result is the same).
+// rec_col_off: byte of (column c, depth d) within a row; det_col_off: byte of the deterministic
+// bucket; rec_off: byte of the reference's bucket index b (c * D + d, the deterministic bucket last)
+// of sketch v.
 template <class G>
 __device__ inline uint32_t rec_col_off(const Geo& g, uint32_t c, uint32_t d) {
-  return c * g.cpitch + d * (uint32_t)(8 + sizeof(G));
+  constexpr uint32_t RB = 8 + sizeof(G);
+  return d < g.low ? (c >> 1) * 128u + (c & 1u) * 64u + d * RB : g.ovf + c * g.opitch + (d - g.low) * RB;
 }
+__device__ inline uint32_t det_col_off(const Geo& g) { return (g.C >> 1) * 128u + (g.C & 1u) * 64u; }
 template <class G>
 __device__ inline size_t rec_off(const Geo& g, uint32_t v, uint32_t b) {
-  const uint32_t c = b == g.nb - 1 ? 0 : b / g.D;
-  const uint32_t d = b == g.nb - 1 ? g.D : b - c * g.D;
+  if (b == g.nb - 1) return (size_t)v * g.row + det_col_off(g);
+  const uint32_t c = b / g.D, d = b - c * g.D;
   return (size_t)v * g.row + rec_col_off<G>(g, c, d);
 }
 template <class G>
@@ -405,7 +406,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
           const uint32_t cc = e / kLdsPitch, d = e - cc * kLdsPitch;
           // entries at the pitch padding (d == kLdsDepth) or at depth >= D are never written;
           // bo is the record's byte offset in the row
-          bo[k] = e == det_slot ? rec_col_off<G>(g, 0, D) : rec_col_off<G>(g, cc, d);
+          bo[k] = e == det_slot ? det_col_off(g) : rec_col_off<G>(g, cc, d);
           da[k] = s_a[e];
           dg[k] = s_g[e];
           if ((da[k] | (uint64_t)dg[k]) != 0) {
@@ -805,7 +806,13 @@ __global__ void k_eager_sf_check(const uint2* __restrict__ pairs, size_t count, 
     if (merged[i]) continue;
     const uint2 p = pairs[i];
     if (p.x == p.y || p.x >= d.n || p.y >= d.n) continue;
-    if (sf_contains(d, concat_pairing(p.x, p.y))) d.flags[kFlagEagerInvalid] = 1;  // same value from every writer
+    if (sf_contains(d, concat_pairing(p.x, p.y))) {
+      d.flags[kFlagEagerInvalid] = 1;  // same value from every writer
+      if (d.inv_mirror) {  // lets the host stop launching eager batches without a synchronisation
+        *d.inv_mirror = 1;
+        __threadfence_system();
+      }
+    }
   }
 }
 
@@ -858,7 +865,7 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
     uint64_t a = 0;
     G gm = 0;
     const bool live = v < g.n && (i < nd || i == GD);
-    if (live) ld_rec<G>(slab + (size_t)v * g.row + (size_t)(i == GD ? g.D : i) * RB, &a, &gm);
+    if (live) ld_rec<G>(slab + (size_t)v * g.row + (i == GD ? det_col_off(g) : rec_col_off<G>(g, 0, i)), &a, &gm);
     const uint64_t mask = __ballot(live && (G)xxh3_mix(xxh3_prep(a), g.chk_bf) == gm);
     const uint32_t gmask = (uint32_t)(mask >> base) & ((1u << GL) - 1);
     const uint32_t colm = gmask & ((1u << nd) - 1);
@@ -886,7 +893,7 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
       const uint32_t dep = GD + lane;  // D - GD <= 64 lanes for every supported n
       uint64_t b = 0;
       G bg = 0;
-      if (dep < g.D) ld_rec<G>(slab + (size_t)u * g.row + (size_t)dep * RB, &b, &bg);
+      if (dep < g.D) ld_rec<G>(slab + (size_t)u * g.row + rec_col_off<G>(g, 0, dep), &b, &bg);
       const uint64_t m2 = __ballot(dep < g.D && (G)xxh3_mix(xxh3_prep(b), g.chk_bf) == bg);
       const uint64_t ia = m2 ? rd_lane_u64(b, __builtin_ctzll(m2)) : 0;
       if (lane == 0) put(u, m2 ? sample_record(kGood, ia, g.n, flags) : kSampFail);

@@ -17,6 +17,7 @@ struct DsuDev {
   uint32_t nmod;
   uint32_t prio_seed;
   uint32_t n;
+  uint32_t* inv_mirror;  // mapped host word set to 1 with kFlagEagerInvalid (nullptr: none)
 };
 
 enum FlagSlot {

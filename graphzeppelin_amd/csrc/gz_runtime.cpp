@@ -42,8 +42,11 @@ static Geo make_geo(uint32_t nsk, uint64_t vector_len, uint64_t seed, size_t sam
   g.cps = (uint32_t)cols;
   g.C = (uint32_t)(samples * cols);
   g.nb = g.C * g.D + 1;
-  g.cpitch = col_pitch_bytes(g.D, 8 + gb);
-  g.row = (uint32_t)row_bytes(g.C, g.D, 8 + gb);
+  const RowLayout L = row_layout(g.C, g.D, 8 + gb);
+  g.low = L.low;
+  g.ovf = L.ovf;
+  g.opitch = L.opitch;
+  g.row = L.row;
   g.Dp = lds_pitch(g.D);
   g.seed = seed;
   g.chk_bf = xxh3_bitflip(seed);
@@ -166,7 +169,7 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const size_t D = calc_bkt_per_col(len), C = calc_cc_samples(n, c.sketches_factor);
   const uint64_t nb = C * D + 1;
   const int gbytes = c.gamma_bytes ? c.gamma_bytes : 4;
-  const uint64_t slab = (uint64_t)n * row_bytes((uint32_t)C, (uint32_t)D, 8 + gbytes);
+  const uint64_t slab = (uint64_t)n * row_layout((uint32_t)C, (uint32_t)D, 8 + gbytes).row;
   const size_t desired = (size_t)(nb * (8 + gbytes) / 4 * c.batch_factor);
   const uint64_t cap = c.flush_threshold ? c.flush_threshold : default_cap_pairs(n, desired);
   const uint32_t t = c.tile ? c.tile : 1024;
@@ -223,6 +226,9 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   dsu.nmod = geo.samples + 2;
   dsu.prio_seed = (uint32_t)(seed * 0x9E3779B97F4A7C15ULL >> 32);
   dsu.n = n;
+  eager_inv.ensure(1);
+  eager_inv.p[0] = 0;
+  dsu.inv_mirror = eager_inv.d;
   launch_fill_u32(flags.p, 0, kNumFlags, st);  // the reset below keeps the bad-input flag
   launch_dsu_reset(dsu, st);
 
@@ -285,6 +291,14 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
 
 GzAlg::~GzAlg() {
   if (st) (void)hipStreamSynchronize(st);
+  if (cst) {
+    (void)hipStreamSynchronize(cst);
+    (void)hipStreamDestroy(cst);
+  }
+  if (copy_ev) (void)hipEventDestroy(copy_ev);
+  if (pend_free_ev) (void)hipEventDestroy(pend_free_ev);
+  for (auto& h : hstage)
+    if (h.ev) (void)hipEventDestroy(h.ev);
   for (auto& t : timed) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -505,16 +519,24 @@ void GzAlg::prof_collect() {
 }
 
 void GzAlg::flush_pending() {
-  if (!pending_n) return;
+  const uint32_t* src = ref_n ? ref_pairs : pending.p;
+  const size_t cnt = ref_n ? ref_n : pending_n;
+  if (!cnt) return;
   const FlushBytes fb{prof_bytes.p, sketch_bytes(), (uint64_t)(geo.C + 1) * (8 + gb)};
-  launch_group(pending.p, pending_n, n, tile, gbufs, st, prof ? &fb : nullptr);
-  if (prof) prof_begin(0, 0, true, 2 * pending_n);
+  launch_group(src, cnt, n, tile, gbufs, st, prof ? &fb : nullptr);
+  if (!ref_n && cst) {  // host copies into `pending` on the copy stream wait until the grouping read it
+    GZ_HIP(hipEventRecord(pend_free_ev, st));
+    pend_wait = true;
+  }
+  if (prof) prof_begin(0, 0, true, 2 * cnt);
   launch_update_tiles(gb, g_tiles.p, g_totals.p + 1, 0, grid_cap, dst_arr.p, nullptr, slab.p,
                       geo, col_bf.p, st);
   if (prof) prof_end();
   GZ_HIP(hipGetLastError());
-  total_updates += 2 * pending_n;
+  total_updates += 2 * cnt;
   pending_n = 0;
+  ref_pairs = nullptr;
+  ref_n = 0;
 }
 
 void GzAlg::ingest_device(const uint32_t* pairs_dev, size_t count) {
@@ -522,10 +544,11 @@ void GzAlg::ingest_device(const uint32_t* pairs_dev, size_t count) {
   if (!count) return;
   std::lock_guard<std::mutex> lk(stage_mtx);
   if (!cfg.eager_dsu) dsu_valid = false;
+  if (ref_n) flush_pending();  // referenced pairs and copied pairs are never mixed in one flush
   while (count) {
     const size_t take = std::min(count, cap_pairs - pending_n);
     uint32_t* dst = pending.p + 2 * pending_n;
-    if (cfg.eager_dsu && dsu_valid) eager_batch(pairs_dev, dst, take, merged.p);  // copies as well
+    if (cfg.eager_dsu && dsu_valid && eager_live()) eager_batch(pairs_dev, dst, take, merged.p);  // copies as well
     else GZ_HIP(hipMemcpyAsync(dst, pairs_dev, take * 8, hipMemcpyDeviceToDevice, st));
     pending_n += take;
     pairs_dev += 2 * take;
@@ -534,22 +557,104 @@ void GzAlg::ingest_device(const uint32_t* pairs_dev, size_t count) {
   }
 }
 
-void GzAlg::ingest_host(const uint32_t* pairs, size_t count) {
+void GzAlg::ingest_device_stable(const uint32_t* pairs_dev, size_t count) {
   if (update_locked) throw GzError(GZ_ERR_UPDATE_LOCKED, "Cannot update the algorithm: Connected components currently running");
   if (!count) return;
   std::lock_guard<std::mutex> lk(stage_mtx);
   if (!cfg.eager_dsu) dsu_valid = false;
+  if (pending_n || (ref_n && pairs_dev != ref_pairs + 2 * ref_n)) flush_pending();
   while (count) {
-    const size_t take = std::min(count, cap_pairs - pending_n);
-    uint32_t* dst = pending.p + 2 * pending_n;
-    GZ_HIP(hipMemcpyAsync(dst, pairs, take * 8, hipMemcpyHostToDevice, st));
+    const size_t take = std::min(count, cap_pairs - ref_n);
+    // the eager batch reads the pairs in place (no copy); once the eager DSU is known to be invalid
+    // (the mapped mirror of its flag) no eager launch is needed until the next query
+    if (cfg.eager_dsu && dsu_valid && eager_live()) eager_batch(pairs_dev, const_cast<uint32_t*>(pairs_dev), take, merged.p);
+    if (!ref_n) ref_pairs = pairs_dev;
+    ref_n += take;
+    pairs_dev += 2 * take;
+    count -= take;
+    if (ref_n == cap_pairs) flush_pending();
+  }
+}
+
+void GzAlg::release_stable() {
+  std::lock_guard<std::mutex> lk(stage_mtx);
+  if (cst) (void)hipStreamSynchronize(cst);  // host pairs read in place have landed
+  if (!ref_n) return;
+  try {
+    flush_pending();
     GZ_HIP(hipStreamSynchronize(st));
-    if (cfg.eager_dsu && dsu_valid) eager_batch(dst, dst, take, merged.p);
+  } catch (...) {
+    ref_pairs = nullptr;
+    ref_n = 0;
+    throw;
+  }
+}
+
+void GzAlg::ensure_host_ingest() {
+  if (cst) return;
+  GZ_HIP(hipStreamCreate(&cst));
+  GZ_HIP(hipEventCreateWithFlags(&copy_ev, hipEventDisableTiming));
+  GZ_HIP(hipEventCreateWithFlags(&pend_free_ev, hipEventDisableTiming));
+  for (auto& h : hstage) {
+    h.buf.alloc(2 * kHostStagePairs);
+    GZ_HIP(hipEventCreateWithFlags(&h.ev, hipEventDisableTiming));
+  }
+  // work already on the compute stream may still read `pending`
+  GZ_HIP(hipEventRecord(pend_free_ev, st));
+  pend_wait = true;
+}
+
+// Whether p lies in page-locked host memory the device can read directly.  Pageable memory either
+// reports an unregistered type or an error, which is cleared here.
+static bool host_pinned(const void* p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+void GzAlg::ingest_host(const uint32_t* pairs, size_t count, bool stable) {
+  if (update_locked) throw GzError(GZ_ERR_UPDATE_LOCKED, "Cannot update the algorithm: Connected components currently running");
+  if (!count) return;
+  std::lock_guard<std::mutex> lk(stage_mtx);
+  if (!cfg.eager_dsu) dsu_valid = false;
+  if (ref_n) flush_pending();
+  ensure_host_ingest();
+  const bool direct = host_pinned(pairs);
+  while (count) {
+    size_t take = std::min(count, cap_pairs - pending_n);
+    const void* src = pairs;
+    HostStage* hs = nullptr;
+    if (!direct) {
+      take = std::min(take, kHostStagePairs);
+      hs = &hstage[hstage_cur];
+      hstage_cur ^= 1;
+      if (hs->busy) GZ_HIP(hipEventSynchronize(hs->ev));  // its previous copy has landed
+      std::memcpy(hs->buf.p, pairs, take * 8);
+      src = hs->buf.p;
+    }
+    uint32_t* dst = pending.p + 2 * pending_n;
+    if (pend_wait) {
+      GZ_HIP(hipStreamWaitEvent(cst, pend_free_ev, 0));
+      pend_wait = false;
+    }
+    GZ_HIP(hipMemcpyAsync(dst, src, take * 8, hipMemcpyHostToDevice, cst));
+    GZ_HIP(hipEventRecord(copy_ev, cst));
+    if (hs) {
+      GZ_HIP(hipEventRecord(hs->ev, cst));
+      hs->busy = true;
+    }
+    GZ_HIP(hipStreamWaitEvent(st, copy_ev, 0));  // the kernels that read `dst` follow the copy
+    if (cfg.eager_dsu && dsu_valid && eager_live()) eager_batch(dst, dst, take, merged.p);
     pending_n += take;
     pairs += 2 * take;
     count -= take;
     if (pending_n == cap_pairs) flush_pending();
   }
+  // pairs read in place are only borrowed for the call
+  if (direct && !stable) GZ_HIP(hipEventSynchronize(copy_ev));
 }
 
 // The driver's force_flush + flush_workers (include/graph_sketch_driver.h:193-203)
@@ -665,7 +770,6 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
   bool answered = false;
   update_locked = true;
   try {
-    const double round_bytes = (double)n * (geo.D + 1) * (8 + gb) + 8.0 * n;
     const bool giant_ok = sketches_sum_to_zero();
     const uint32_t last_round = geo.samples - 1;  // round index `samples` would be out of samples
     if (hround_pending) {
@@ -722,11 +826,7 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
       fin_r0 = hi + 1;
       next = hi + 1;
     }
-    if (prof) {
-      prof_end();
-      timed.back().host_bytes = round_bytes * (double)(rounds + 1);
-      timed.back().items = (uint64_t)rounds + 1;
-    }
+    if (prof) prof_end();
     last_rounds = (size_t)rounds;
     last_finish_start = hf[dsu.nmod - 1] ? fin_r0 : 0;
     if (reported) {
@@ -737,8 +837,23 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
       GZ_HIP(hipEventRecord(hround_ev, st));
       hround_on_event = true;
     }
+    if (prof) {
+      // algorithmic bytes of the query: round 0 reads depths 0..14 of column 0 and the deterministic
+      // bucket of every vertex and writes a sample record; round r >= 1 reads column r and the
+      // deterministic bucket ((D+1) records) and a root and a member word of every vertex it
+      // regroups (those outside a giant supernode, as recorded per round)
+      if (hround_on_event) GZ_HIP(hipEventSynchronize(hround_ev));
+      const double rec = 8.0 + gb;
+      double bytes = (double)n * (std::min<uint32_t>(geo.D, 15) + 1) * rec + 8.0 * n;
+      for (int64_t r = 1; r <= rounds && r < 64; ++r) bytes += (double)hround.p[2 * r] * ((geo.D + 1) * rec + 8.0);
+      timed.back().host_bytes = bytes;
+      timed.back().items = (uint64_t)rounds + 1;
+    }
     hround_pending = true;
     sf_pending = cfg.eager_dsu;
+    // every eager batch enqueued before the query has finished (the query ended with a stream
+    // synchronisation), so none can set the mirror after this
+    eager_inv.p[0] = 0;
     dsu_valid = true;
     if (prof) prof_collect();
   } catch (...) {

@@ -175,7 +175,9 @@ class GzAlg {
   RoundBufs rb{};
   // device stream ingest (pending pairs -> grouping -> tiles)
   size_t cap_pairs;
-  size_t pending_n = 0;
+  size_t pending_n = 0;  // pairs copied into `pending`
+  const uint32_t* ref_pairs = nullptr;  // pairs referenced in the caller's memory (ingest_device_stable)
+  size_t ref_n = 0;
   DevBuf<uint32_t> pending, dst_arr, g_cnt, g_off, g_toff, g_cursor, g_totals, g_hist, g_hoff, g_htotals;
   DevBuf<uint2> g_rec;
   DevBuf<uint64_t> g_hpartials;
@@ -201,6 +203,20 @@ class GzAlg {
   size_t stage_cap;
   std::vector<uint32_t> vstamp, vtiles;
   uint32_t epoch = 0;
+  // host ingest: copy stream, pinned staging (two buffers of kHostStagePairs pairs)
+  static constexpr size_t kHostStagePairs = size_t(1) << 20;
+  hipStream_t cst = nullptr;
+  struct HostStage {
+    PinBuf<uint32_t> buf;
+    hipEvent_t ev = nullptr;  // recorded on the copy stream after the buffer's last copy
+    bool busy = false;
+  };
+  HostStage hstage[2];
+  int hstage_cur = 0;
+  hipEvent_t copy_ev = nullptr;  // the latest host copy; the compute stream waits on it
+  hipEvent_t pend_free_ev = nullptr;  // a flush's grouping has read `pending` (host copies may overwrite it)
+  bool pend_wait = false;             // the next host copy into `pending` must wait for pend_free_ev
+  void ensure_host_ingest();
   // eager staging (pre_insert)
   std::mutex eager_mtx;
   PinBuf<uint32_t> eager_host;
@@ -210,6 +226,8 @@ class GzAlg {
   size_t eager_used = 0, eager_cap;
   PinBuf<uint32_t> hflags;  // device flags, or a query's round flags
   PinBuf<uint32_t> hsmall;
+  MapBuf<uint32_t> eager_inv;  // mapped mirror of kFlagEagerInvalid, cleared after each query
+  bool eager_live() const { return !(eager_inv.p && *reinterpret_cast<const volatile uint32_t*>(eager_inv.p)); }
   MapBuf<uint32_t> aq_in;  // a query group's point queries: us then vs, read by the answer kernel
   MapBuf<uint32_t> aq_out;  // the error flags, then one answer byte per point query, written by it
 
@@ -253,7 +271,19 @@ class GzAlg {
   void apply_update_batch(uint32_t src, const uint32_t* dsts, size_t k);
   void apply_raw(uint32_t src, const void* raw);
   void ingest_device(const uint32_t* pairs_dev, size_t count);
-  void ingest_host(const uint32_t* pairs, size_t count);
+  // ingest_device for a caller whose pairs stay valid and unchanged until release_stable(): the
+  // flush groups them where they are (consecutive calls on consecutive ranges of one buffer form
+  // one range), and the eager DSU reads them in place, so nothing is copied per call.
+  void ingest_device_stable(const uint32_t* pairs_dev, size_t count);
+  // Applies (or, if that fails, forgets) every referenced range: after it returns nothing reads
+  // memory given to ingest_device_stable.
+  void release_stable();
+  // (src,dst) pairs in host memory.  Each chunk is DMA'd into the pending buffer on a copy stream
+  // that overlaps the kernels: straight from the caller's memory when it is pinned, else through
+  // two pinned staging buffers.  stable: the caller keeps the pairs valid and unchanged until the
+  // next flush, so the call need not wait for its DMA; otherwise a call whose pairs were read in
+  // place returns only after their copy has landed.
+  void ingest_host(const uint32_t* pairs, size_t count, bool stable = false);
   bool has_cached_query(int code);
   void prep_query(int code);
   uint32_t connected_components(uint32_t* parent_out);

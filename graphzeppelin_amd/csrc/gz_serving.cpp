@@ -52,6 +52,9 @@ static void answer_group(GzAlg& A, const gz_request* reqs, size_t i, size_t j, u
   st.queries += j - i;
 }
 
+void serve_trace_body(GzAlg& A, const uint32_t* pairs, bool on_device, const gz_request* reqs, size_t nreq,
+                      double* lat, uint32_t* answers, gz_serve_stats& st);
+
 static void check_requests(const GzAlg& A, const gz_request* reqs, size_t nreq) {
   for (size_t i = 0; i < nreq; ++i) {
     const gz_request& r = reqs[i];
@@ -60,17 +63,29 @@ static void check_requests(const GzAlg& A, const gz_request* reqs, size_t nreq) 
   }
 }
 
+// The stream buffer stays valid for the whole call, so device pairs are ingested by reference
+// (grouped where they are at the next flush); every reference is released before returning.
 void serve_trace(GzAlg& A, const uint32_t* pairs, bool on_device, const gz_request* reqs, size_t nreq, double* lat,
                  uint32_t* answers, gz_serve_stats& st) {
   check_requests(A, reqs, nreq);
+  try {
+    serve_trace_body(A, pairs, on_device, reqs, nreq, lat, answers, st);
+  } catch (...) {
+    A.release_stable();
+    throw;
+  }
+}
+
+void serve_trace_body(GzAlg& A, const uint32_t* pairs, bool on_device, const gz_request* reqs, size_t nreq,
+                      double* lat, uint32_t* answers, gz_serve_stats& st) {
   st = gz_serve_stats{};
   const auto t0 = Clock::now();
   size_t i = 0;
   while (i < nreq) {
     if (reqs[i].kind == GZ_REQ_UPDATES) {
       const uint32_t* p = pairs + 2 * reqs[i].offset;
-      if (on_device) A.ingest_device(p, reqs[i].count);
-      else A.ingest_host(p, reqs[i].count);
+      if (on_device) A.ingest_device_stable(p, reqs[i].count);
+      else A.ingest_host(p, reqs[i].count, /*stable=*/true);
       st.updates += reqs[i].count;
       if (lat) lat[i] = 0;
       if (answers) answers[i] = 0;

@@ -34,6 +34,9 @@
 #include <utility>
 #include <vector>
 
+#include <fcntl.h>
+#include <unistd.h>
+
 #include "gz_hip.h"
 
 namespace gzamd {
@@ -263,6 +266,9 @@ class CCSketchAlg {
   void print_configuration() { std::cout << config << std::endl; }
   // Device ingest of whole update chunks (pre_insert + both directions): lets the driver skip the
   // host gutters.  Not part of the reference interface; GraphSketchDriver uses it when present.
+  // Device ingest of (src, dst) pairs in host memory (pre_insert + both directions), DMA'd
+  // asynchronously; the pairs are borrowed for the call only.
+  void ingest_pairs(const uint32_t* pairs, size_t count) { check(gz_alg_ingest_pairs(h_, pairs, count)); }
   void ingest_updates(const GraphStreamUpdate* upd, size_t count) {
     std::vector<uint32_t> pairs(2 * count);
     for (size_t i = 0; i < count; ++i) {
@@ -360,29 +366,73 @@ class MemoryStream : public GraphStream {
   }
 };
 
-// Binary stream file (README.md:57-72): u32 vertices | u64 updates | 9-byte {u8 type, u32 src, u32 dst}
-class BinaryFileStream : public MemoryStream {
+// Binary stream file (README.md:57-72): u32 vertices | u64 updates | 9-byte {u8 type, u32 src, u32 dst}.
+// Read from the file as the stream threads ask for updates: each get_update_buffer call reserves
+// the next run of records under a lock and reads them with pread, so several reader threads read
+// disjoint parts of the file at once (thread safe, like StreamingUtilities' BinaryFileStream).
+class BinaryFileStream : public GraphStream {
+  int fd = -1;
+  std::mutex mtx;
+  edge_id_t pos = 0, brk = END_OF_STREAM;
+  static constexpr uint64_t kHeader = 12, kRecord = 9;
+
  public:
   explicit BinaryFileStream(const std::string& path) {
-    std::ifstream f(path, std::ios::binary);
-    if (!f) throw DriverException("cannot open " + path);
+    fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) throw DriverException("cannot open " + path);
+    unsigned char h[kHeader];
+    if (::pread(fd, h, kHeader, 0) != (ssize_t)kHeader) {
+      ::close(fd);
+      throw DriverException("truncated stream header " + path);
+    }
     uint32_t n = 0;
     uint64_t m = 0;
-    f.read((char*)&n, 4);
-    f.read((char*)&m, 8);
-    std::vector<unsigned char> raw(m * 9);
-    f.read((char*)raw.data(), (std::streamsize)raw.size());
-    if ((uint64_t)f.gcount() != raw.size()) throw DriverException("truncated stream " + path);
-    upd.resize(m);
-    for (uint64_t i = 0; i < m; ++i) {
-      const unsigned char* r = &raw[9 * i];
-      uint32_t s, d;
-      std::memcpy(&s, r + 1, 4);
-      std::memcpy(&d, r + 5, 4);
-      upd[i] = GraphStreamUpdate{Edge{s, d}, r[0]};
-    }
+    std::memcpy(&n, h, 4);
+    std::memcpy(&m, h + 4, 8);
     num_vertices = n;
     num_edges = m;
+  }
+  BinaryFileStream(const BinaryFileStream&) = delete;
+  BinaryFileStream& operator=(const BinaryFileStream&) = delete;
+  ~BinaryFileStream() override {
+    if (fd >= 0) ::close(fd);
+  }
+  size_t get_update_buffer(GraphStreamUpdate* buf, size_t n) override {
+    edge_id_t start = 0, cnt = 0;
+    bool at_end = false;
+    {
+      std::lock_guard<std::mutex> lk(mtx);
+      const edge_id_t end = std::min<edge_id_t>(brk, num_edges);
+      start = pos;
+      cnt = std::min<edge_id_t>(n, end > pos ? end - pos : 0);
+      pos += cnt;
+      at_end = pos >= end;
+    }
+    if (cnt) {
+      std::vector<unsigned char> raw(cnt * kRecord);
+      size_t got = 0;
+      while (got < raw.size()) {
+        const ssize_t r = ::pread(fd, raw.data() + got, raw.size() - got, (off_t)(kHeader + start * kRecord + got));
+        if (r <= 0) throw DriverException("stream file read failed");
+        got += (size_t)r;
+      }
+      for (edge_id_t i = 0; i < cnt; ++i) {
+        const unsigned char* r = &raw[kRecord * i];
+        uint32_t a, b;
+        std::memcpy(&a, r + 1, 4);
+        std::memcpy(&b, r + 5, 4);
+        buf[i] = GraphStreamUpdate{Edge{a, b}, r[0]};
+      }
+    }
+    size_t k = (size_t)cnt;
+    if (k < n && at_end) buf[k++] = GraphStreamUpdate{Edge{}, (uint8_t)BREAKPOINT};
+    return k;
+  }
+  bool set_break_point(edge_id_t idx) override {
+    std::lock_guard<std::mutex> lk(mtx);
+    if (idx != END_OF_STREAM && idx < pos) return false;
+    brk = idx;
+    return true;
   }
 };
 
@@ -433,6 +483,15 @@ struct has_ingest : std::false_type {};
 template <class A>
 struct has_ingest<A, decltype(std::declval<A&>().ingest_updates((const GraphStreamUpdate*)nullptr, size_t(0)), void())>
     : std::true_type {};
+template <class A, class = void>
+struct has_ingest_pairs : std::false_type {};
+template <class A>
+struct has_ingest_pairs<A, decltype(std::declval<A&>().ingest_pairs((const uint32_t*)nullptr, size_t(0)), void())>
+    : std::true_type {};
+template <class A, class = void>
+struct has_prep_query : std::false_type {};
+template <class A>
+struct has_prep_query<A, decltype(std::declval<A&>().prep_query(0), void())> : std::true_type {};
 }  // namespace detail
 
 // GraphSketchDriver<Alg> with the reference's interface.  With DEVICE gutters (the default) and an
@@ -465,12 +524,28 @@ class GraphSketchDriver {
     batch_callback(thr, src, full);
   }
 
+  // A reader thread's host gutter: its updates as (src, dst) pairs, handed to the device whole.
+  static constexpr size_t reader_batch = size_t(1) << 20;
   template <class A = Alg>
-  typename std::enable_if<detail::has_ingest<A>::value>::type device_ingest(const GraphStreamUpdate* u, size_t k) {
-    alg->ingest_updates(u, k);
+  typename std::enable_if<detail::has_ingest_pairs<A>::value>::type device_ingest(const std::vector<uint32_t>& p) {
+    alg->ingest_pairs(p.data(), p.size() / 2);
   }
   template <class A = Alg>
-  typename std::enable_if<!detail::has_ingest<A>::value>::type device_ingest(const GraphStreamUpdate*, size_t) {}
+  typename std::enable_if<!detail::has_ingest_pairs<A>::value && detail::has_ingest<A>::value>::type device_ingest(
+      const std::vector<uint32_t>& p) {
+    std::vector<GraphStreamUpdate> u(p.size() / 2);
+    for (size_t i = 0; i < u.size(); ++i) u[i] = GraphStreamUpdate{Edge{p[2 * i], p[2 * i + 1]}, (uint8_t)INSERT};
+    alg->ingest_updates(u.data(), u.size());
+  }
+  template <class A = Alg>
+  typename std::enable_if<!detail::has_ingest_pairs<A>::value && !detail::has_ingest<A>::value>::type device_ingest(
+      const std::vector<uint32_t>&) {}
+  template <class A = Alg>
+  typename std::enable_if<detail::has_prep_query<A>::value>::type device_flush(int code) {
+    alg->prep_query(code);
+  }
+  template <class A = Alg>
+  typename std::enable_if<!detail::has_prep_query<A>::value>::type device_flush(int) {}
 
  public:
   GraphSketchDriver(Alg* a, GraphStream* s, DriverConfiguration c = DriverConfiguration(), size_t n_stream_threads = 1)
@@ -478,7 +553,7 @@ class GraphSketchDriver {
         gmtx(a->get_num_vertices()) {
     alg->allocate_worker_memory(conf.get_worker_threads());
     gutter_size = std::max<size_t>(1, alg->get_desired_updates_per_batch());
-    device_path = conf.get_gutter_sys() == DEVICE && detail::has_ingest<Alg>::value;
+    device_path = conf.get_gutter_sys() == DEVICE && (detail::has_ingest<Alg>::value || detail::has_ingest_pairs<Alg>::value);
     if (num_stream_threads > 1 && !stream->get_update_is_thread_safe()) num_stream_threads = 1;
     alg->print_configuration();
   }
@@ -490,6 +565,15 @@ class GraphSketchDriver {
     std::mutex dev_mtx;
     auto task = [&](int thr) {
       std::vector<GraphStreamUpdate> buf(update_array_size);
+      std::vector<uint32_t> gutter;  // device path: this reader's pairs, handed over when full
+      if (device_path) gutter.reserve(2 * reader_batch);
+      auto hand_over = [&] {
+        if (gutter.empty()) return;
+        std::lock_guard<std::mutex> lk(dev_mtx);
+        device_ingest(gutter);
+        total_updates += gutter.size();  // two directed updates per pair
+        gutter.clear();
+      };
       while (true) {
         size_t k = stream->get_update_buffer(buf.data(), update_array_size);
         size_t m = 0;
@@ -497,9 +581,11 @@ class GraphSketchDriver {
         while (m < k && buf[m].type != BREAKPOINT) ++m;
         if (m < k) stop = true;
         if (device_path) {
-          std::lock_guard<std::mutex> lk(dev_mtx);
-          device_ingest(buf.data(), m);
-          total_updates += 2 * m;
+          for (size_t i = 0; i < m; ++i) {
+            gutter.push_back(buf[i].edge.src);
+            gutter.push_back(buf[i].edge.dst);
+          }
+          if (gutter.size() >= 2 * reader_batch) hand_over();
         } else {
           for (size_t i = 0; i < m; ++i) {
             GraphUpdate upd{buf[i].edge, (UpdateType)buf[i].type};
@@ -508,7 +594,10 @@ class GraphSketchDriver {
             host_insert(upd.edge.dst, upd.edge.src, thr);
           }
         }
-        if (stop || k == 0) return;
+        if (stop || k == 0) {
+          hand_over();
+          return;
+        }
       }
     };
     std::vector<std::thread> th;
@@ -520,6 +609,7 @@ class GraphSketchDriver {
   void prep_query(int query_code) {
     flush_start = flush_end = std::chrono::steady_clock::now();
     if (alg->has_cached_query(query_code)) return;
+    if (device_path) device_flush(query_code);  // the device gutters: group and apply every pending update
     if (!device_path)
       for (node_id_t v = 0; v < gutters.size(); ++v)
         if (!gutters[v].empty()) {

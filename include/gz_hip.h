@@ -161,7 +161,8 @@ uint64_t gz_alg_get_total_updates(const gz_alg* a);
  * include/cc_sketch_alg.h:250-252).  When enabled, HIP events on the instance's stream bracket every
  * sketch-update launch and every Borůvka round; alg_bytes is the algorithmic HBM traffic of those
  * launches: per vertex batch of k destinations 4k + 2*min(S, k*(C+1)*B) with S the sketch bytes,
- * C the columns and B the bucket bytes; per Borůvka round n*(D+1)*B + 8n. */
+ * C the columns and B the bucket bytes; per Borůvka query n*(min(D,15)+1)*B + 8n for round 0 and
+ * e_r*((D+1)*B + 8) for round r >= 1, e_r the vertices round r regroups. */
 typedef struct {
   uint64_t launches;
   double total_ms;
