@@ -857,6 +857,7 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
     }
     samp[v] = rec;
   };
+  if (ld_acq(d.flags + kFlagSkip)) return;  // the query keeps the eager DSU
   const uint32_t lane = lane_id(), i = lane % GL, base = lane - i;
   const uint32_t nd = g.D < GD ? g.D : GD;  // depths checked in the grouped pass
   const uint32_t waves = (gridDim.x * blockDim.x) / kWave;
@@ -931,10 +932,25 @@ constexpr uint32_t kMaxStatRounds = 64;
 
 // cnt / st (a query's start, or nullptr): the root counts are cleared too, and the round state
 // words are set to their start values (no giant), so a query begins with one launch.
+//
+// force (a query's start, or nullptr): when it reads 0 and the eager DSU is still valid, the query
+// keeps that DSU -- its answer is the cached one -- so the kernel only clears the round flags (every
+// round kernel of the query then returns at once), sets the round state words to their start values
+// and sets kFlagSkip.  The eager-invalid flag is only read here (every workgroup sees the same value)
+// and is cleared by round 0 of a query that runs.
 __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uint64_t sf_cap, uint64_t* sf_edge,
-                            uint32_t* flags, uint32_t* mod, uint32_t nmod, uint32_t* cnt, uint32_t* st) {
+                            uint32_t* flags, uint32_t* mod, uint32_t nmod, uint32_t* cnt, uint32_t* st,
+                            const uint32_t* force) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (force && !*force && !ld_acq(flags + kFlagEagerInvalid)) {
+    if (t == 0) flags[kFlagSkip] = 1;
+    for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
+    // The round kernels read their walk counts and member lists before they look at the round
+    // flag, so the state words get their start values (empty walks) here as well.
+    if (st && t < kStWords) st[t] = (t == kStGiantRound || t == kStGiantFirst) ? kNoRoot : 0u;
+    return;
+  }
   for (size_t i = t; i < n; i += stride) {
     parent[i] = (uint32_t)i;
     sf_edge[i] = kEmpty;
@@ -942,8 +958,9 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
   }
   for (size_t i = t; i < sf_cap; i += stride) sf_hash[i] = kEmpty;
   // The bad-input flag is left alone: pairs grouped by the flush just before a query set it, and the
-  // query's answer reports it (it is cleared where it is reported).
-  if (t < kNumFlags && t != kFlagBadInput) flags[t] = 0;
+  // query's answer reports it (it is cleared where it is reported).  A query's reset leaves the
+  // eager-invalid flag to round 0 (see above).
+  if (t < kNumFlags && t != kFlagBadInput && !(force && t == kFlagEagerInvalid)) flags[t] = 0;
   for (size_t i = t; i < nmod; i += stride) mod[i] = 0;
   if (st && t < kStWords) st[t] = (t == kStGiantRound || t == kStGiantFirst) ? kNoRoot : 0u;
 }
@@ -976,7 +993,11 @@ k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ 
                 const uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
                 const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod,
                 const unsigned char* __restrict__ hint) {
-  const uint32_t live = prev_mod ? ld_acq(prev_mod) : 1u;  // checked after the first reads
+  uint32_t live = prev_mod ? ld_acq(prev_mod) : 1u;  // checked after the first reads
+  if (!st) {  // round 0: nothing to do when the query keeps the eager DSU; else the DSU restarts valid
+    if (ld_acq(d.flags + kFlagSkip)) live = 0;
+    else if (blockIdx.x == 0 && threadIdx.x == 0) d.flags[kFlagEagerInvalid] = 0;
+  }
   const bool act = st && st[kStActMode] != 0;
   const uint32_t gr = act ? st[kStGiantRound] : kNoRoot;
   const uint32_t count = st ? walk_count(st, round, d.n) : d.n;
@@ -1106,7 +1127,7 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
 
 __global__ void __launch_bounds__(kAggBlock)
 k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
-             uint32_t* __restrict__ members, const uint32_t* __restrict__ st, uint32_t round,
+             uint32_t* __restrict__ members, uint32_t* __restrict__ mroot, const uint32_t* __restrict__ st, uint32_t round,
              const uint32_t* __restrict__ act_list, const uint32_t* __restrict__ prev_mod) {
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads (see k_bv_find_count)
   const uint32_t gr = st[kStGiantRound];
@@ -1132,7 +1153,11 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
   for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
     if (t_key[k] != kEmptyKey) t_base[k] = atomicAdd(cursor + t_key[k], t_cnt[k]);
   __syncthreads();
-  if (active) members[t_base[my_slot] + my_pos] = v;
+  if (active) {  // the root beside the member, so the reduce reads both in list order
+    const uint32_t p = t_base[my_slot] + my_pos;
+    members[p] = v;
+    mroot[p] = r;
+  }
 }
 
 // Segmented XOR-reduce + sample (range_merge of column `round` + the deterministic bucket over
@@ -1334,7 +1359,7 @@ __device__ inline void xacc_add(uint64_t* slot, uint64_t a, uint64_t gm) {
 template <class G>
 __global__ void __launch_bounds__(kRedBlock)
 k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ members,
-            const uint32_t* __restrict__ vroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
+            const uint32_t* __restrict__ mroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
             XAcc xa, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
             const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats,
             const unsigned char* __restrict__ det_cpy, uint32_t small_walk) {
@@ -1362,7 +1387,7 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
   const uint32_t nblk = (e1 + span - 1) / span;
   const uint32_t b0 = blk * span;
   const uint32_t b1 = b0 + span < e1 ? b0 + span : e1;
-  auto root_at = [&](uint32_t p) -> uint32_t { return vroot[members[p]]; };
+  auto root_at = [&](uint32_t p) -> uint32_t { return mroot[p]; };
   const bool giant_tot = giant != kNoRoot;
   // neighbours of the workgroup's range, for phase 2; loaded early so they are in flight with phase 1
   const uint32_t blk_prev = (wid == 0 && b0 > 0) ? root_at(b0 - 1) : kNoRoot;
@@ -1379,8 +1404,8 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
     const uint32_t prev = p0 == 0 ? kNoRoot : root_at(p0 - 1);
     const uint32_t next = p0 + len >= e1 ? kNoRoot : root_at(p0 + len);
     uint64_t wa, wg;
-    reduce_range<G, true>(p0, len, prev, next, members, vroot, slab, g, round, nullptr, nullptr, hs, ts, samp, flags,
-                          &wa, &wg, det_cpy, &xa, b0, b1, s_bnd);
+    reduce_range<G, true, true>(p0, len, prev, next, members, nullptr, slab, g, round, nullptr, nullptr, hs, ts, samp,
+                                flags, &wa, &wg, det_cpy, &xa, b0, b1, s_bnd, mroot);
     if (giant_tot) {
       lds_xor(s_tot + lane, wa);
       lds_xor(s_tot + kWave + lane, wg);
@@ -1925,7 +1950,15 @@ void launch_dsu_reset(const DsuDev& d, hipStream_t s, const RoundBufs* rb) {
   const size_t work = std::max<size_t>(d.n, d.sf_mask + 1);
   hipLaunchKernelGGL(k_dsu_reset, dim3(grid_for(work, 256)), dim3(256), 0, s, d.parent, d.n, d.sf_hash,
                      d.sf_mask + 1, d.sf_edge, d.flags, d.mod, d.nmod, rb ? rb->cnt : nullptr,
-                     rb ? rb->totals : nullptr);
+                     rb ? rb->totals : nullptr, rb ? d.query_force : nullptr);
+}
+
+void launch_round_state_init(const RoundBufs& rb, uint32_t n, hipStream_t s) {
+  launch_fill_u32(rb.totals, 0, kStWords, s);
+  launch_fill_u32(rb.totals + kStGiantRound, kNoRoot, 1, s);
+  launch_fill_u32(rb.totals + kStGiantFirst, kNoRoot, 1, s);
+  launch_fill_u32(rb.members[0], 0, n, s);
+  launch_fill_u32(rb.members[1], 0, n, s);
 }
 
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
@@ -1961,17 +1994,17 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
                      act_list, prev);
   hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, rb.cnt, n, rb.off, rb.cursor, st, round, act_list,
                      giant_ok ? 1u : 0u, prev);
-  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, st, round,
-                     act_list, prev);
+  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, rb.mroot, st,
+                     round, act_list, prev);
   // one accumulator slot per workgroup and one for the giant
   const uint32_t rgrid = reduce_blocks(n);
   if (rgrid + 1 > rb.xslots) throw std::logic_error("reduce grid exceeds its cross-workgroup accumulator slots");
   const XAcc xa{rb.xacc, rb.xcnt, rb.xslots, rb.off, rb.cursor};
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.root,
+    hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot,
                        slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
   else
-    hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.root,
+    hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot,
                        slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
                      act_list, prev, cur, rb.hint);

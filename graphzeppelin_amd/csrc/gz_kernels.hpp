@@ -18,12 +18,15 @@ struct DsuDev {
   uint32_t prio_seed;
   uint32_t n;
   uint32_t* inv_mirror;  // mapped host word set to 1 with kFlagEagerInvalid (nullptr: none)
+  // Mapped host word read at a query's start: nonzero, or a set kFlagEagerInvalid, runs Borůvka;
+  // otherwise the query keeps the valid eager DSU (nullptr: always run).
+  const uint32_t* query_force;
 };
 
 enum FlagSlot {
   kFlagModified = 0,
   kFlagEagerInvalid = 1,
-  kFlagUnused = 2,
+  kFlagSkip = 2,      // the last query found the eager DSU valid and left it in place (no Borůvka)
   kFlagBad = 3,       // a sample decoded to an out-of-range vertex
   kFlagBadInput = 4,  // an ingested pair named a vertex >= n (the pair was skipped)
   kNumFlags = 8
@@ -93,6 +96,7 @@ struct RoundBufs {
   uint32_t* off;         // n      segment start per root
   uint32_t* cursor;      // n      scatter cursor per root (the segment end after the scatter)
   uint32_t* members[2];  // n each vertices grouped by root, by round parity (also the next active list)
+  uint32_t* mroot;       // n      the root of members[round & 1][p] at position p (written by the scatter)
   uint32_t* totals;      // 8      round state words (entry counts, giant roots, walk mode)
   uint64_t* xacc;        // xslots * 128: cross-workgroup supernode sums, zero between rounds
   uint32_t* xcnt;        // xslots arrival counters, zero between rounds
@@ -102,6 +106,9 @@ struct RoundBufs {
   unsigned char* det_cpy;  // n records: every vertex's deterministic bucket, copied in round 0
   unsigned char* hint;     // n      round 0: each vertex's degree class, the first key of its merge order
 };
+// Start values of the round state words (no giant, empty walks) and in-range member lists, for a new
+// instance: round kernels read both before they look at their round flag.
+void launch_round_state_init(const RoundBufs& rb, uint32_t n, hipStream_t s);
 // Writes rb.samp: a sample record per vertex.
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
                           hipStream_t s);

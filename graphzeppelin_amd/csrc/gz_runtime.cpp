@@ -176,10 +176,10 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint64_t part = n <= (1u << 24) ? cap * 16 /*records*/ + 8 * 1048577ull + (1 << 16) : 0;
   const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ + part +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
-  // per vertex: root, cnt, off, cursor, 2 member lists, parent, labels (4 B each), sample record,
+  // per vertex: root, cnt, off, cursor, 2 member lists, member roots, parent, labels (4 B each), sample record,
   // forest edge (8 each), spanning-forest hash set (16), deterministic-bucket copy, degree class
   // (1); the reduce's cross-workgroup accumulators (1 KiB + 4 B per 1024 vertices, and one more slot)
-  const uint64_t rounds = (uint64_t)n * (8 * 4 + 8 + 8 + 16 + 8 + gbytes + 1) + ((uint64_t)n / 512 + 3) * 1028 + 8192;
+  const uint64_t rounds = (uint64_t)n * (9 * 4 + 8 + 8 + 16 + 8 + gbytes + 1) + ((uint64_t)n / 512 + 3) * 1028 + 8192;
   return slab + ingest + rounds;
 }
 
@@ -229,12 +229,15 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   eager_inv.ensure(1);
   eager_inv.p[0] = 0;
   dsu.inv_mirror = eager_inv.d;
+  qforce.ensure(1);
+  qforce.p[0] = 1;
+  dsu.query_force = qforce.d;
   launch_fill_u32(flags.p, 0, kNumFlags, st);  // the reset below keeps the bad-input flag
   launch_dsu_reset(dsu, st);
 
   // Borůvka round buffers
   r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
-  r_members[0].alloc(n); r_members[1].alloc(n);
+  r_members[0].alloc(n); r_members[1].alloc(n); r_mroot.alloc(n);
   r_totals.alloc(8);
   r_stats.alloc(128);
   r_det.alloc((size_t)n * (8 + gb));
@@ -246,9 +249,13 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   r_xcnt.alloc(xslots);
   launch_fill_u64(r_xacc.p, 0, xslots * 128, st);
   launch_fill_u32(r_xcnt.p, 0, xslots, st);
-  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_totals.p,
+  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p}, r_mroot.p, r_totals.p,
                  r_xacc.p, r_xcnt.p, (uint32_t)xslots, r_samp.p, r_stats.p, r_det.p, r_hint.p};
   launch_fill_u32(r_cnt.p, 0, n, st);
+  // Round kernels read the state words and the previous member list before their round flag, so
+  // both hold in-range values from the start (empty walks, vertex 0); every query's first kernel
+  // sets the state words again.
+  launch_round_state_init(rb, n, st);
 
   // device ingest
   cap_pairs = cfg.flush_threshold ? (size_t)cfg.flush_threshold : default_cap_pairs(n, desired_updates_per_batch());
@@ -766,8 +773,11 @@ hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
 // once per group: first the graph prefix (round 0, rounds 1..K and a finish attempt), then groups
 // of 2 rounds, each followed by a finish attempt.  A finish that takes over runs every remaining
 // round, so at most one does per query.
-bool GzAlg::boruvka(const std::function<void()>* speculate) {
+bool GzAlg::boruvka(const std::function<void()>* speculate, bool gated, bool* skipped) {
   bool answered = false;
+  if (skipped) *skipped = false;
+  gated = gated && speculate;  // the skip is read from the speculated answers' flags
+  qforce.p[0] = gated ? 0u : 1u;  // read by the query's first kernel, after this store
   update_locked = true;
   try {
     const bool giant_ok = sketches_sum_to_zero();
@@ -787,6 +797,7 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
     }
     const uint32_t want = next_prefix ? next_prefix : kFirstGraphRounds;
     const uint32_t first = std::max(1u, std::min(want, last_round));
+    const bool sf_was_pending = sf_pending;
     sf_pending = false;  // the DSU reset clears the set
     if (prof) prof_begin(1, 0, false, 0);
     if (use_graphs) GZ_HIP(hipGraphLaunch(get_query_graph(giant_ok, first), st));
@@ -812,6 +823,18 @@ bool GzAlg::boruvka(const std::function<void()>* speculate) {
           rounds = r;
           break;
         }
+      if (first_group && gated && aq_out.p[kFlagSkip]) {
+        // the eager DSU was still valid: the query's kernels kept it, and the answers came from it
+        if (prof) {
+          ev_pool.push_back(timed.back().a);
+          ev_pool.push_back(timed.back().b);
+          timed.pop_back();
+        }
+        if (skipped) *skipped = true;
+        sf_pending = sf_was_pending;  // no reset ran: the forest set is as it was
+        update_locked = false;
+        return true;
+      }
       if (rounds >= 0) {
         answered = first_group && speculate;
         break;
@@ -909,8 +932,19 @@ bool GzAlg::answer_queries(const uint32_t* us, const uint32_t* vs, size_t npoint
   for (size_t i = 0; i < npoint; ++i)
     if (us[i] >= n || vs[i] >= n) throw GzError(GZ_ERR_INVALID_ARG, "vertex id out of range");
   std::lock_guard<std::mutex> lk(query_mtx);
-  sync_eager_state();
-  const bool cached = dsu_valid;
+  // Whether the eager DSU still answers is decided on the device when the host cannot tell: the
+  // query is enqueued gated (flush, then a Borůvka that keeps a valid eager DSU), so no host
+  // synchronisation precedes it.  Host-side pre_insert calls are applied first.
+  bool may_be_cached = false;
+  if (cfg.eager_dsu) {
+    {
+      std::lock_guard<std::mutex> g(eager_mtx);
+      if (dsu_valid) submit_eager_locked();
+      else eager_used = 0;
+    }
+    may_be_cached = dsu_valid && eager_live();
+  }
+  bool cached = false;
   // The pairs go to mapped host memory, where the answer kernel reads them and leaves the error
   // flags and the answers (no copy launches around it).  The previous group's kernel has finished:
   // every group ends with a stream synchronisation.
@@ -929,16 +963,16 @@ bool GzAlg::answer_queries(const uint32_t* us, const uint32_t* vs, size_t npoint
     }
   };
   bool answered = false;
-  if (!cached) {
-    {
-      std::lock_guard<std::mutex> g(stage_mtx);
-      if (cfg.batched) submit_stage_locked();
-      flush_pending();
-    }
-    // The answers are enqueued with the first group of rounds; they stand when that group
-    // finished the query (the usual case), else they are enqueued again after the last round.
-    answered = boruvka(&enqueue_answers);
+  {
+    std::lock_guard<std::mutex> g(stage_mtx);
+    if (cfg.batched) submit_stage_locked();
+    flush_pending();
   }
+  // The answers are enqueued with the first group of rounds; they stand when that group finished
+  // the query (the usual case), else they are enqueued again after the last round.
+  bool skipped = false;
+  answered = boruvka(&enqueue_answers, may_be_cached, &skipped);
+  cached = skipped;
   if (!answered) {
     enqueue_answers();
     GZ_HIP(hipStreamSynchronize(st));

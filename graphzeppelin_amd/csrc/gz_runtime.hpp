@@ -168,7 +168,7 @@ class GzAlg {
   DevBuf<uint8_t> qout;
   DsuDev dsu{};
   // Borůvka round buffers
-  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_totals, r_stats, mod;
+  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_mroot, r_totals, r_stats, mod;
   DevBuf<uint64_t> r_xacc, r_samp;
   DevBuf<uint8_t> r_det, r_hint;
   DevBuf<uint32_t> r_xcnt;
@@ -227,6 +227,7 @@ class GzAlg {
   PinBuf<uint32_t> hflags;  // device flags, or a query's round flags
   PinBuf<uint32_t> hsmall;
   MapBuf<uint32_t> eager_inv;  // mapped mirror of kFlagEagerInvalid, cleared after each query
+  MapBuf<uint32_t> qforce;     // DsuDev::query_force: 1 runs the next query whatever the eager DSU's state
   bool eager_live() const { return !(eager_inv.p && *reinterpret_cast<const volatile uint32_t*>(eager_inv.p)); }
   MapBuf<uint32_t> aq_in;  // a query group's point queries: us then vs, read by the answer kernel
   MapBuf<uint32_t> aq_out;  // the error flags, then one answer byte per point query, written by it
@@ -308,7 +309,10 @@ class GzAlg {
   // reading the DSU); it runs right after the first group of rounds, and boruvka() returns true when
   // that group finished the query, so the work read the final DSU and one synchronisation served
   // both; false means it has to be enqueued again.
-  bool boruvka(const std::function<void()>* speculate = nullptr);
+  // gated (with speculate): the device runs the query only if the eager DSU turned invalid;
+  // otherwise every kernel of it returns at once, the answers come from the eager DSU, and
+  // *skipped is set.
+  bool boruvka(const std::function<void()>* speculate = nullptr, bool gated = false, bool* skipped = nullptr);
   uint32_t read_flag(int slot);
 
   // Query prefix (DSU reset, round 0, rounds 1..K and a single-workgroup finish of the remaining

@@ -237,17 +237,17 @@ def multi_tenant_trace(tenants, seed: int, rate_updates_per_s: float | None, cc_
     requests (on average cc_per_tenant connected_components() requests per tenant, at uniformly
     random stream positions, plus as many u-v point queries times point_frac / (1 - point_frac)),
     and every tenant ends with one connected_components() request.  Arrival times: every tenant's
-    updates arrive at the same steady rate, so that all tenants together offer rate_updates_per_s
-    (None: everything arrives at time 0); a query arrives with the batch before it.  Requests are
-    ordered by arrival time."""
+    stream arrives at a steady rate over the same span, total updates / rate_updates_per_s, so the
+    tenants together offer rate_updates_per_s throughout (None: everything arrives at time 0); a
+    query arrives with the batch before it.  Requests are ordered by arrival time."""
     from ._lib import REQ_CC, REQ_POINT, REQ_UPDATES, REQUEST_DTYPE
 
     rng = np.random.default_rng(seed)
     rows = []
-    T = len(tenants)
+    span = None if rate_updates_per_s is None else sum(int(p.shape[0]) for _, p in tenants) / rate_updates_per_s
     for t, (n, pairs) in enumerate(tenants):
         total = int(pairs.shape[0])
-        per_tenant_rate = None if rate_updates_per_s is None else rate_updates_per_s / T
+        per_tenant_rate = None if span is None else total / span
         qrate = (cc_per_tenant / (1.0 - point_frac)) / total  # queries per update
         off = 0
         while off < total:

@@ -55,7 +55,9 @@ def test_multi_tenant_trace_shape():
         # the stream arrives once, in order, at the tenant's steady rate
         assert upd["offset"][0] == 0 and np.array_equal(upd["offset"][1:], np.cumsum(upd["count"])[:-1])
         assert int(upd["count"].sum()) == p.shape[0]
-        assert np.allclose(upd["arrival_s"], (upd["offset"] + upd["count"]) / (1e6 / 5))
+        span = sum(q.shape[0] for _, q in tenants) / 1e6  # every stream arrives over the same span
+        assert np.allclose(upd["arrival_s"], (upd["offset"] + upd["count"]) / (p.shape[0] / span))
+        assert np.isclose(upd["arrival_s"][-1], span)
         assert sub[-1]["kind"] == gz.REQ_CC  # every stream ends with a connected_components() request
         pts = sub[sub["kind"] == gz.REQ_POINT]
         assert np.all(pts["u"] < n) and np.all(pts["v"] < n)

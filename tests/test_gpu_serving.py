@@ -101,6 +101,62 @@ def test_out_of_range_device_pair_is_reported_by_a_query(eager):
     assert np.array_equal(alg.dump_sketches(), ref.dump())
 
 
+def test_cached_first_query_on_recycled_memory():
+    """A query the eager DSU still answers keeps that DSU and skips every Borůvka kernel.  On a new
+    instance whose device memory was just used and freed by another instance (so it is not zero),
+    the round kernels of such a first query must still only read in-range state; answers exact."""
+    import torch
+
+    say("cached first query on recycled memory")
+    n = 4096
+    pairs, _ = streams.dynamic_erdos_stream(n, 0.002, 0.3, 0.1, 2, 31)
+    dev = torch.from_numpy(pairs.astype(np.int32)).cuda()
+    dirty = gz.CCSketchAlg(n, 5, gz.CCAlgConfiguration(eager_dsu=False))
+    dirty.serve_trace(dev.data_ptr(), True, trace_for(n, pairs.shape[0], 9, 3000))
+    dirty.close()
+    # insert-only prefix: the eager DSU is valid at the first query, which is answered from it
+    ins = pairs[:2000]
+    trace = np.array([(gz.REQ_UPDATES, 0, 0, ins.shape[0], 0, 0, 0.0), (gz.REQ_CC, 0, 0, 0, 0, 0, 0.0),
+                      (gz.REQ_POINT, 0, 0, 0, int(ins[0, 0]), int(ins[0, 1]), 0.0),
+                      (gz.REQ_UPDATES, 0, ins.shape[0], pairs.shape[0] - ins.shape[0], 0, 0, 0.0),
+                      (gz.REQ_CC, 0, 0, 0, 0, 0, 0.0)], dtype=gz.REQUEST_DTYPE)
+    alg = gz.CCSketchAlg(n, 5)
+    _, ans, st = alg.serve_trace(dev.data_ptr(), True, trace)
+    assert st["cached_answers"] >= 1
+    c1, _ = oracle.exact_cc(n, ins)
+    c2, _ = oracle.exact_cc(n, pairs)
+    assert list(ans[[1, 2, 4]]) == [c1, 1, c2]
+    ref = oracle.CCAlg(n, 5)
+    ref.ingest_pairs(pairs)
+    assert np.array_equal(alg.dump_sketches(), ref.dump())
+
+
+def test_query_kept_cached_right_after_borůvka_keeps_forest_checks():
+    """Two query calls with no update between: the first runs Borůvka (its reset empties the
+    spanning-forest set, which the next eager batch refills), the second keeps the fresh DSU.  The
+    second must leave the refill pending, or later deletions of forest edges would go unnoticed:
+    deleting every live edge afterwards must give n components."""
+    import torch
+
+    say("cached query right after Borůvka")
+    n = 2048
+    pairs, _ = streams.dynamic_erdos_stream(n, 0.003, 0.3, 0.1, 2, 41)
+    live = streams.final_edge_set(pairs)
+    alg = gz.CCSketchAlg(n, 8)
+    dev = torch.from_numpy(pairs.astype(np.int32)).cuda()
+    t1 = np.array([(gz.REQ_UPDATES, 0, 0, pairs.shape[0], 0, 0, 0.0), (gz.REQ_CC, 0, 0, 0, 0, 0, 0.0)],
+                  dtype=gz.REQUEST_DTYPE)
+    _, a1, s1 = alg.serve_trace(dev.data_ptr(), True, t1)
+    c, _ = oracle.exact_cc(n, pairs)
+    assert a1[1] == c and s1["boruvka_runs"] == 1  # the stream's deletions invalidated the eager DSU
+    dels = torch.from_numpy(live.astype(np.int32)).cuda()
+    t2 = np.array([(gz.REQ_CC, 0, 0, 0, 0, 0, 0.0), (gz.REQ_UPDATES, 0, 0, live.shape[0], 0, 0, 0.0),
+                   (gz.REQ_CC, 0, 0, 0, 0, 0, 0.0)], dtype=gz.REQUEST_DTYPE)
+    _, a2, s2 = alg.serve_trace(dels.data_ptr(), True, t2)
+    assert a2[0] == c and s2["cached_answers"] >= 1  # kept the DSU Borůvka had just built
+    assert a2[2] == n  # every edge deleted
+
+
 def test_plain_and_batched_forms_agree():
     say("plain vs batched n=2048")
     n = 2048
@@ -163,6 +219,21 @@ def _mixed_trace(tenants, seed, per_req=20000):
     return arr
 
 
+def _exact_answers(n, pairs_dev, trace):
+    import exact_cc
+
+    life = exact_cc.EdgeLifetimes(pairs_dev)
+    out = np.zeros(trace.size, dtype=np.uint32)
+    upto = 0
+    for i, r in enumerate(trace):
+        if r["kind"] == gz.REQ_UPDATES:
+            upto = int(r["offset"] + r["count"])
+            continue
+        c, lab = life.components(n, upto)
+        out[i] = c if r["kind"] == gz.REQ_CC else int(lab[r["u"]] == lab[r["v"]])
+    return out
+
+
 def test_multi_tenant_mix_equals_alone():
     say("multi-tenant mix")
     tenants = _tenant_streams(6, 11)
@@ -183,6 +254,7 @@ def test_multi_tenant_mix_equals_alone():
         alone = gz.CCSketchAlg(n, seeds[t], cfg)
         alat, aans, ast = alone.serve_trace(p.data_ptr(), True, sub)
         sq = sub["kind"] != 0
+        assert np.array_equal(aans[sq], _exact_answers(n, p, sub)[sq]), f"tenant {t} alone"
         assert np.array_equal(ans[sel][sq], aans[sq]), f"tenant {t}"
         assert int(dig[t]) == alone.digest(), f"tenant {t}"
         assert np.all(lat[sel][sq] >= 0)
