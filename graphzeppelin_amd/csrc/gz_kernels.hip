@@ -43,6 +43,22 @@ __device__ inline uint64_t wave_xor_u64(uint64_t v) {
   }
   return v;
 }
+// XOR over the wave of a 32-bit value, returned to every lane: DPP row shifts build each row's XOR
+// in its lane 15, two row broadcasts carry it to lane 63 (no LDS round trips).  Every lane must be
+// active.
+__device__ inline uint32_t wave_xor_dpp_u32(uint32_t v) {
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1);
+}
+__device__ inline uint64_t wave_xor_dpp_u64(uint64_t v) {
+  const uint32_t lo = wave_xor_dpp_u32((uint32_t)v), hi = wave_xor_dpp_u32((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
 __device__ inline uint64_t wave_incl_scan_u64(uint64_t x) {
   const uint32_t lane = lane_id();
   for (int o = 1; o < kWave; o <<= 1) {
@@ -300,13 +316,17 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   uint64_t* s_bf = reinterpret_cast<uint64_t*>(smem);
   uint64_t* s_a = s_bf + C;
   G* s_g = reinterpret_cast<G*>(s_a + lds_n);
+  uint32_t* s_bo = reinterpret_cast<uint32_t*>(s_g + lds_n);  // each delta entry's byte in the row
   const uint32_t lane = threadIdx.x;
 
   for (uint32_t i = lane; i < C; i += kWave) s_bf[i] = col_bf[i];
-  // The delta is zeroed once here; afterwards the write-back clears every entry it reads.
+  // The delta is zeroed once here; afterwards the write-back clears every entry it reads.  Entries
+  // at the pitch padding (d == kLdsDepth) or at depth >= D are never written: offset 0.
   for (uint32_t i = lane; i < lds_n; i += kWave) {
     s_a[i] = 0;
     s_g[i] = 0;
+    const uint32_t cc = i / kLdsPitch, d = i - cc * kLdsPitch;
+    s_bo[i] = i == det_slot ? det_col_off(g) : (d < kLdsDepth && d < D ? rec_col_off<G>(g, cc, d) : 0u);
   }
   const uint32_t ntiles = ntiles_dev ? *ntiles_dev : ntiles_host;
   const uint64_t depth_cap = 1ull << D;
@@ -375,8 +395,8 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
         }
       }
     }
-    det_a = wave_xor_u64(det_a);
-    det_g = (G)wave_xor_u64((uint64_t)det_g);
+    det_a = wave_xor_dpp_u64(det_a);
+    det_g = sizeof(G) == 4 ? (G)wave_xor_dpp_u32((uint32_t)det_g) : (G)wave_xor_dpp_u64((uint64_t)det_g);
     if (lane == 0 && (det_a | (uint64_t)det_g)) {
       s_a[det_slot] ^= det_a;
       s_g[det_slot] ^= det_g;
@@ -403,10 +423,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
         dg[k] = 0;
         bo[k] = 0;
         if (e < lds_n) {
-          const uint32_t cc = e / kLdsPitch, d = e - cc * kLdsPitch;
-          // entries at the pitch padding (d == kLdsDepth) or at depth >= D are never written;
-          // bo is the record's byte offset in the row
-          bo[k] = e == det_slot ? det_col_off(g) : rec_col_off<G>(g, cc, d);
+          bo[k] = s_bo[e];
           da[k] = s_a[e];
           dg[k] = s_g[e];
           if ((da[k] | (uint64_t)dg[k]) != 0) {
@@ -1621,13 +1638,17 @@ k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ ac
       rstats[2 * r + 1] = gcur;
     }
     __syncthreads();
-    // reduce + sample (k_bv_reduce): one wave per chunk of 64 entries
-    const uint32_t nchunks = (e + kWave - 1) / kWave;
+    // reduce + sample (k_bv_reduce): one wave per chunk of cw <= 64 entries, the entries spread
+    // over all waves (at least 8 per chunk), so a small round is a short chain of gathers; at most
+    // 16 chunks below 64 entries each, at most kFinishChunks of 64
+    const uint32_t per_wave = (e + kRedWaves - 1) / kRedWaves;
+    const uint32_t cw = per_wave >= kWave ? kWave : (per_wave < 8 ? 8u : (per_wave + 7) & ~7u);
+    const uint32_t nchunks = (e + cw - 1) / cw;
     Slot hs{s_slot + (size_t)(2 * wid) * 2 * kWave, s_slotroot + 2 * wid};
     Slot ts{s_slot + (size_t)(2 * wid + 1) * 2 * kWave, s_slotroot + 2 * wid + 1};
     for (uint32_t c = wid; c < nchunks; c += kRedWaves) {
-      const uint32_t p0 = c * kWave;
-      const uint32_t len = e - p0 < kWave ? e - p0 : kWave;
+      const uint32_t p0 = c * cw;
+      const uint32_t len = e - p0 < cw ? e - p0 : cw;
       const uint32_t prev = p0 > 0 ? s_mroot[p0 - 1] : kNoRoot;
       const uint32_t next = p0 + len < e ? s_mroot[p0 + len] : kNoRoot;
       uint64_t wa, wg;
@@ -1637,7 +1658,7 @@ k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ ac
       lds_xor(s_tot + kWave + lane, wg);
       const uint32_t hr = *hs.root, tr = *ts.root;
       if (hr != kNoRoot) {  // started in an earlier chunk
-        uint64_t* acc = s_acc + (size_t)(s_seg[p0] / kWave) * 2 * kWave;
+        uint64_t* acc = s_acc + (size_t)(s_seg[p0] / cw) * 2 * kWave;
         lds_xor(acc + lane, hs.vals[lane]);
         lds_xor(acc + kWave + lane, hs.vals[kWave + lane]);
       }
@@ -1870,7 +1891,7 @@ void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) {
 
 size_t update_tiles_lds_bytes(const Geo& g, int gbytes) {
   const size_t lds_n = (size_t)g.C * kLdsPitch + 1;
-  return g.C * 8 + lds_n * 8 + lds_n * (size_t)gbytes;
+  return g.C * 8 + lds_n * 8 + lds_n * (size_t)gbytes + lds_n * 4;
 }
 
 void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
