@@ -339,15 +339,20 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   // checksum once; the chunk's (update, column) pairs then fetch those from the owning lane with
   // shuffles, so each pair costs one column hash and no memory round trip.  The next tile's
   // descriptor and each lane's first index are loaded before the current tile's write-back.
-  auto load_idx = [&](const Tile& tt, uint32_t u) -> uint64_t {
-    return RAW ? raw_idx[tt.begin + u] : concat_pairing(tt.vertex, dsts[tt.begin + u]);
+  // The prefetched word is the raw destination (or raw index): the pairing is formed when the
+  // tile starts, so waiting for the load does not precede the write-back.
+  auto load_raw = [&](const Tile& tt, uint32_t u) -> uint64_t {
+    return RAW ? raw_idx[tt.begin + u] : (uint64_t)dsts[tt.begin + u];
+  };
+  auto to_idx = [&](const Tile& tt, uint64_t w) -> uint64_t {
+    return RAW ? w : concat_pairing(tt.vertex, (uint32_t)w);
   };
   uint32_t t = blockIdx.x;
   Tile tl{0, 0, 0, 0};
-  uint64_t idx0 = 0;
+  uint64_t raw0 = 0;
   if (t < ntiles) {
     tl = tiles[t];
-    if (lane < tl.count) idx0 = load_idx(tl, lane);
+    if (lane < tl.count) raw0 = load_raw(tl, lane);
   }
   while (t < ntiles) {
     unsigned char* row = slab + (size_t)tl.vertex * g.row;
@@ -359,7 +364,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
       uint64_t my_pre = 0;
       G my_chk = 0;
       if (lane < m) {
-        const uint64_t idx = u0 == 0 ? idx0 : load_idx(tl, u0 + lane);
+        const uint64_t idx = to_idx(tl, u0 == 0 ? raw0 : load_raw(tl, u0 + lane));
         my_pre = xxh3_prep(idx);
         my_chk = (G)xxh3_mix(my_pre, g.chk_bf);
         det_a ^= idx;
@@ -405,10 +410,10 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
 
     const uint32_t tn = t + gridDim.x;
     Tile tl_next{0, 0, 0, 0};
-    uint64_t idx0_next = 0;
+    uint64_t raw0_next = 0;
     if (tn < ntiles) {
       tl_next = tiles[tn];
-      if (lane < tl_next.count) idx0_next = load_idx(tl_next, lane);
+      if (lane < tl_next.count) raw0_next = load_raw(tl_next, lane);
     }
 
     // write-back: read-xor-write of the touched records, four LDS entries per lane in flight
@@ -451,7 +456,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     __syncthreads();
     t = tn;
     tl = tl_next;
-    idx0 = idx0_next;
+    raw0 = raw0_next;
   }
 }
 
