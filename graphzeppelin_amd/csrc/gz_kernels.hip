@@ -313,7 +313,9 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   const uint32_t C = g.C, D = g.D;
   const uint32_t det_slot = C * kLdsPitch;
   const uint32_t lds_n = det_slot + 1;
-  uint64_t* s_bf = reinterpret_cast<uint64_t*>(smem);
+  // per chunk: each update's pre-mixed index and checksum, read by the lanes that take its columns
+  ulonglong2* s_pc = reinterpret_cast<ulonglong2*>(smem);
+  uint64_t* s_bf = reinterpret_cast<uint64_t*>(smem + kWave * sizeof(ulonglong2));
   uint64_t* s_a = s_bf + C;
   G* s_g = reinterpret_cast<G*>(s_a + lds_n);
   uint32_t* s_bo = reinterpret_cast<uint32_t*>(s_g + lds_n);  // each delta entry's byte in the row
@@ -336,9 +338,10 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   __syncthreads();
 
   // Updates are taken in chunks of 64: lane l loads update u0 + l once and hashes its index and
-  // checksum once; the chunk's (update, column) pairs then fetch those from the owning lane with
-  // shuffles, so each pair costs one column hash and no memory round trip.  The next tile's
-  // descriptor and each lane's first index are loaded before the current tile's write-back.
+  // checksum once into a 16-byte LDS slot; the chunk's (update, column) pairs then read their
+  // update's slot with one LDS load, so each pair costs one column hash and no memory round trip.
+  // The next tile's descriptor and each lane's first index are loaded before the current tile's
+  // write-back.
   // The prefetched word is the raw destination (or raw index): the pairing is formed when the
   // tile starts, so waiting for the load does not precede the write-back.
   auto load_raw = [&](const Tile& tt, uint32_t u) -> uint64_t {
@@ -370,15 +373,15 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
         det_a ^= idx;
         det_g ^= my_chk;
       }
+      s_pc[lane] = make_ulonglong2(my_pre, (uint64_t)my_chk);  // the wave's LDS ops stay in order
       const uint32_t np = m * C;
       const uint32_t iters = (np + kWave - 1) / kWave;  // uniform over the wave
       uint32_t q = i0, c = c0;
       for (uint32_t it = 0; it < iters; ++it) {
-        // shuffles with the whole wave active; q < 64 always
-        const uint64_t pre = shfl_u64(my_pre, (int)(q & (kWave - 1)));
-        const G chk = sizeof(G) == 4 ? (G)__shfl((unsigned)(uint32_t)my_chk, (int)(q & (kWave - 1)))
-                                     : (G)shfl_u64((uint64_t)my_chk, (int)(q & (kWave - 1)));
-        if (it * kWave + lane < np) {
+        if (it * kWave + lane < np) {  // q < m here
+          const ulonglong2 pc = s_pc[q];
+          const uint64_t pre = pc.x;
+          const G chk = (G)pc.y;
           const uint64_t idx = (pre << 32) | (pre >> 32);  // xxh3_prep swaps the halves
           const uint64_t h = xxh3_mix(pre, s_bf[c]);
           const uint32_t depth = (uint32_t)__builtin_ctzll(h | depth_cap);  // == D: no bucket
@@ -1896,7 +1899,7 @@ void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) {
 
 size_t update_tiles_lds_bytes(const Geo& g, int gbytes) {
   const size_t lds_n = (size_t)g.C * kLdsPitch + 1;
-  return g.C * 8 + lds_n * 8 + lds_n * (size_t)gbytes + lds_n * 4;
+  return kWave * 16 + g.C * 8 + lds_n * 8 + lds_n * (size_t)gbytes + lds_n * 4;
 }
 
 void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_dev, uint32_t ntiles_host,
