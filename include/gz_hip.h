@@ -125,7 +125,10 @@ int gz_alg_update(gz_alg* a, uint32_t src, uint32_t dst, int type);
  * include/graph_sketch_driver.h:154-178): `count` updates given as (src,dst) uint32 pairs in device
  * memory, each applied to both endpoints' sketches and passed through pre_insert. */
 int gz_alg_ingest_pairs_device(gz_alg* a, const uint32_t* pairs_device, size_t count);
-/* The same for (src,dst) pairs in host memory. */
+/* The same for (src,dst) pairs in host memory.  The copy to the device runs on a stream of its own
+ * and overlaps the kernels of earlier batches; pinned memory (hipHostMalloc) is copied directly,
+ * pageable memory through pinned staging buffers.  `pairs` is borrowed for the call only: the call
+ * returns once the pairs have been read. */
 int gz_alg_ingest_pairs(gz_alg* a, const uint32_t* pairs, size_t count);
 /* has_cached_query(query_code), include/cc_sketch_alg.h:184-190 -> *out 0/1 */
 int gz_alg_has_cached_query(gz_alg* a, int query_code, int* out);

@@ -42,7 +42,9 @@ typedef struct {
 } gz_serve_stats;
 
 /* Batched/served form over one algorithm instance.  `pairs` holds the stream as (src,dst) uint32
- * pairs (device memory when pairs_on_device != 0, else host memory).  Consecutive queries form one
+ * pairs (device memory when pairs_on_device != 0, else host memory); it must stay unchanged until
+ * the call returns, since batches are read in place (device memory, pinned host memory) or copied
+ * asynchronously (pageable host memory, through pinned staging).  Consecutive queries form one
  * group answered against one state (at most one Borůvka); each request's latency is measured from
  * the moment the loop reaches it to its answer.  latency_s / answers: nreq entries (updates get 0;
  * point queries 0/1; CC queries the component count). */
