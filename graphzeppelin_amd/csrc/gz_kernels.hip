@@ -950,8 +950,9 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot,
 //                     vertices of (count << 32 | root); the next round excludes that supernode.
 //                     The complement sum needs no majority -- any supernode's sum is the XOR of the
 //                     vertices outside it -- so a large one can be left out a round early.
-constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStBest = 6;
-constexpr uint32_t kStWords = 8;
+//   kStNseg + (r & 1) supernodes outside the giant in round r (entries of RoundBufs::segs[r & 1])
+constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStBest = 6, kStNseg = 8;
+constexpr uint32_t kStWords = 10;
 constexpr uint32_t kCandidateDiv = 32;
 constexpr uint32_t kMaxStatRounds = 64;
 
@@ -1011,13 +1012,12 @@ __device__ inline bool apply_sample(const DsuDev& d, uint64_t s, const unsigned 
   return true;
 }
 
-// st == nullptr (round 0): every vertex is a root.  Otherwise the round's walk (every vertex, or
-// the active list plus the giant's root, which the list does not hold).
+// st == nullptr (round 0): every vertex is a root.  Otherwise the round's supernodes: its segment
+// list (every root outside the giant) and the giant's root.
 __global__ void __launch_bounds__(256)
-k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ root, DsuDev d,
-                const uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
-                const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ cur_mod,
-                const unsigned char* __restrict__ hint) {
+k_apply_samples(const uint64_t* __restrict__ samp, const RoundSeg* __restrict__ segs, DsuDev d,
+                const uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ prev_mod,
+                uint32_t* __restrict__ cur_mod, const unsigned char* __restrict__ hint) {
   uint32_t live = prev_mod ? ld_acq(prev_mod) : 1u;  // checked after the first reads
   if (!st) {  // round 0: nothing to do when the query keeps the eager DSU; else the DSU restarts valid
     if (ld_acq(d.flags + kFlagSkip)) live = 0;
@@ -1025,12 +1025,11 @@ k_apply_samples(const uint64_t* __restrict__ samp, const uint32_t* __restrict__ 
   }
   const bool act = st && st[kStActMode] != 0;
   const uint32_t gr = act ? st[kStGiantRound] : kNoRoot;
-  const uint32_t count = st ? walk_count(st, round, d.n) : d.n;
+  const uint32_t count = st ? st[kStNseg + (round & 1)] : d.n;
   if (!live) return;
   bool modified = false;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-    const uint32_t v = walk_vertex(act_list, round, i);
-    if (root && root[v] != v) continue;
+    const uint32_t v = st ? segs[i].root : i;
     if (v == gr) continue;  // handled once below
     if (apply_sample(d, samp[v], hint)) modified = true;
   }
@@ -1085,6 +1084,7 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
   const uint32_t gcur = act ? dsu_find_stable(parent, gfirst) : kNoRoot;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st[kStSeg + (round & 1)] = 0;
+    st[kStNseg + (round & 1)] = 0;
     st[kStGiantRound] = gcur;
     st[kStGiantFirst] = gfirst;
     st[kStActMode] = act ? 1u : 0u;
@@ -1110,13 +1110,17 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
 // Segments for the roots met in this round's walk (every non-giant root is a member of its own
 // supernode, so it is walked).  cnt is cleared here, its last reader.  Without a giant yet, a root
 // holding more than half of the vertices is elected: at most one can.
+// A root's segment goes into the round's segment list too (block scan of the counts and of the
+// roots, packed in one 64-bit scan: counts in the low 40 bits, roots above).
+constexpr uint32_t kSegShift = 40;
+
 __global__ void __launch_bounds__(kAggBlock)
 k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, uint32_t* __restrict__ cursor,
            uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list, uint32_t giant_ok,
-           const uint32_t* __restrict__ prev_mod) {
+           const uint32_t* __restrict__ prev_mod, RoundSeg* __restrict__ segs) {
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads (see k_bv_find_count)
   __shared__ uint64_t wsum[kAggBlock / kWave];
-  __shared__ uint32_t s_base;
+  __shared__ uint32_t s_base, s_sbase;
   const bool act = st[kStActMode] != 0;
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
@@ -1133,7 +1137,8 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
   } else if (!act && giant_ok && c >= 2 && c >= n / kCandidateDiv) {  // at most kCandidateDiv roots
     atomicMax(reinterpret_cast<unsigned long long*>(st + kStBest), ((unsigned long long)c << 32) | v);
   }
-  const uint64_t incl = wave_incl_scan_u64(c);
+  const uint64_t x = c ? ((uint64_t)1 << kSegShift) | c : 0;
+  const uint64_t incl = wave_incl_scan_u64(x);
   if (lane == kWave - 1) wsum[wid] = incl;
   __syncthreads();
   uint64_t pre = 0, tot = 0;
@@ -1141,12 +1146,19 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
     if (w < wid) pre += wsum[w];
     tot += wsum[w];
   }
-  if (threadIdx.x == 0) s_base = tot ? atomicAdd(st + kStSeg + (round & 1), (uint32_t)tot) : 0;
+  constexpr uint64_t kCntMask = ((uint64_t)1 << kSegShift) - 1;
+  if (threadIdx.x == 0) {
+    s_base = (tot & kCntMask) ? atomicAdd(st + kStSeg + (round & 1), (uint32_t)(tot & kCntMask)) : 0;
+    s_sbase = (tot >> kSegShift) ? atomicAdd(st + kStNseg + (round & 1), (uint32_t)(tot >> kSegShift)) : 0;
+  }
   __syncthreads();
   if (c) {
-    const uint32_t o = s_base + (uint32_t)(pre + incl - c);
+    const uint64_t ex = pre + incl - x;
+    const uint32_t o = s_base + (uint32_t)(ex & kCntMask);
+    const uint32_t k = s_sbase + (uint32_t)(ex >> kSegShift);
     off[v] = o;
     cursor[v] = o;
+    if (k < n) segs[k] = RoundSeg{v, o, c, 0};
   }
 }
 
@@ -1183,6 +1195,150 @@ k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict
     members[p] = v;
     mroot[p] = r;
   }
+}
+
+// ---- regrouping by segments (rounds >= 2) ------------------------------------------------------
+// A supernode of round r is a union of supernodes of round r - 1, and those are contiguous runs
+// ("segments") of round r - 1's member list.  So round r regroups whole runs: A) the new root of
+// every old segment, and the run's offset inside its new root's segment, taken from the root's
+// member counter (atomics aggregated per block in LDS); B) a contiguous segment per new root and
+// the round's segment list; C) one pass over the old member list copying each member to its place
+// (its root's segment start + its run's offset + its place in the run), with its new root beside
+// it and its parent pointer compressed to that root.  A and B walk the old segments (one per old
+// supernode), only C walks the members, in list order.  Runs whose new root is the giant are dropped, as the vertex walk drops
+// the giant's members.  Every kernel loads its counts and lists before it looks at the round flag
+// (all in range: they are those of an earlier round of this query, or the start values) and writes
+// nothing when the round is not live; indices computed from device data are bounded by n before
+// every store.
+
+__global__ void __launch_bounds__(kAggBlock)
+k_seg_find(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ cnt, uint2* __restrict__ nd,
+           uint32_t* __restrict__ st, uint32_t round, const RoundSeg* __restrict__ segs_old,
+           const uint32_t* __restrict__ prev_mod) {
+  const uint32_t live = ld_acq(prev_mod);
+  uint32_t gfirst = st[kStGiantFirst];
+  const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
+  const uint32_t nseg = st[kStNseg + ((round - 1) & 1)];
+  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool in = i < nseg;
+  RoundSeg sg{0, 0, 0, 0};
+  if (in) sg = segs_old[i];
+  // block 0 writes the round state even with no segments to walk
+  if (!live || (blockIdx.x > 0 && blockIdx.x * kAggBlock >= nseg)) return;
+  __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable];
+  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock) {
+    t_key[k] = kEmptyKey;
+    t_cnt[k] = 0;
+  }
+  if (gfirst == kNoRoot && best) gfirst = (uint32_t)best;  // a candidate recorded by the previous round
+  const bool act = gfirst != kNoRoot;
+  const uint32_t gcur = act ? dsu_find_stable(parent, gfirst) : kNoRoot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[kStSeg + (round & 1)] = 0;
+    st[kStNseg + (round & 1)] = 0;
+    st[kStGiantRound] = gcur;
+    st[kStGiantFirst] = gfirst;
+    st[kStActMode] = act ? 1u : 0u;
+  }
+  __shared__ uint32_t t_base[kAggTable];
+  const bool ok = in && sg.root < n;
+  uint32_t r = 0;
+  if (ok) r = dsu_find_stable(parent, sg.root);
+  __syncthreads();
+  const bool counted = ok && r != gcur;
+  uint32_t my_slot = 0, my_pos = 0;
+  if (ok) parent[sg.root] = r;  // path compression of the old root (members are compressed in C)
+  if (counted) {
+    my_slot = lds_table_slot(t_key, r);
+    my_pos = atomicAdd(t_cnt + my_slot, sg.len);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
+    if (t_key[k] != kEmptyKey) t_base[k] = atomicAdd(cnt + t_key[k], t_cnt[k]);
+  __syncthreads();
+  // the run's offset in its new root's segment, minus its old start: a member at old position i
+  // goes to off[new root] + i + nd.y
+  if (ok) nd[sg.root] = make_uint2(r, counted ? t_base[my_slot] + my_pos - sg.start : 0u);
+}
+
+// B: a segment for every new root (an old root that is still a root) outside the giant; cnt is
+// cleared here, its last reader.  Giant election and candidate as in k_bv_alloc.
+__global__ void __launch_bounds__(kAggBlock)
+k_seg_alloc(uint32_t* __restrict__ cnt, uint32_t n, const uint2* __restrict__ nd, uint32_t* __restrict__ off,
+            uint32_t* __restrict__ cursor, RoundSeg* __restrict__ segs_new, uint32_t* __restrict__ st, uint32_t round,
+            const RoundSeg* __restrict__ segs_old, uint32_t giant_ok, const uint32_t* __restrict__ prev_mod) {
+  const uint32_t live = ld_acq(prev_mod);
+  __shared__ uint64_t wsum[kAggBlock / kWave];
+  __shared__ uint32_t s_base, s_sbase;
+  const bool act = st[kStActMode] != 0;
+  const uint32_t nseg = st[kStNseg + ((round - 1) & 1)];
+  const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
+  const bool in = i < nseg;
+  const uint32_t v = in ? segs_old[i].root : 0;
+  const bool ok = in && v < n;
+  const bool is_root = ok && nd[v].x == v;
+  uint32_t c = is_root ? cnt[v] : 0;  // zero for the giant's root: its members are not counted
+  if (!live || blockIdx.x * kAggBlock >= nseg) return;  // whole blocks past the segments
+  if (c) cnt[v] = 0;
+  if (!act && giant_ok && 2ull * c > n) {
+    st[kStGiantFirst] = v;
+    st[kStGiantRound] = v;
+    c = 0;
+  } else if (!act && giant_ok && c >= 2 && c >= n / kCandidateDiv) {
+    atomicMax(reinterpret_cast<unsigned long long*>(st + kStBest), ((unsigned long long)c << 32) | v);
+  }
+  const uint64_t x = c ? ((uint64_t)1 << kSegShift) | c : 0;
+  const uint64_t incl = wave_incl_scan_u64(x);
+  if (lane == kWave - 1) wsum[wid] = incl;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+  for (uint32_t w = 0; w < kAggBlock / kWave; ++w) {
+    if (w < wid) pre += wsum[w];
+    tot += wsum[w];
+  }
+  constexpr uint64_t kCntMask = ((uint64_t)1 << kSegShift) - 1;
+  if (threadIdx.x == 0) {
+    s_base = (tot & kCntMask) ? atomicAdd(st + kStSeg + (round & 1), (uint32_t)(tot & kCntMask)) : 0;
+    s_sbase = (tot >> kSegShift) ? atomicAdd(st + kStNseg + (round & 1), (uint32_t)(tot >> kSegShift)) : 0;
+  }
+  __syncthreads();
+  if (c) {
+    const uint64_t ex = pre + incl - x;
+    const uint32_t o = s_base + (uint32_t)(ex & kCntMask);
+    const uint32_t k = s_sbase + (uint32_t)(ex >> kSegShift);
+    off[v] = o;
+    cursor[v] = o + c;  // the segment end, as the vertex scatter leaves it
+    if (k < n) segs_new[k] = RoundSeg{v, o, c, 0};
+  }
+}
+
+// C: every member of the old list to its place, its new root beside it, its parent compressed to
+// that root.
+__global__ void __launch_bounds__(256)
+k_seg_copy(uint32_t n, const uint2* __restrict__ nd, const uint32_t* __restrict__ off, uint32_t* __restrict__ parent,
+           const uint32_t* __restrict__ members_old, const uint32_t* __restrict__ mroot_old,
+           uint32_t* __restrict__ members_new, uint32_t* __restrict__ mroot_new, const uint32_t* __restrict__ st,
+           uint32_t round, const uint32_t* __restrict__ prev_mod) {
+  const uint32_t live = ld_acq(prev_mod);
+  const uint32_t gr = st[kStGiantRound];
+  const uint32_t e_old = st[kStSeg + ((round - 1) & 1)];
+  const uint32_t e_new = st[kStSeg + (round & 1)];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < e_old && i < n;
+  uint32_t v = 0, R = 0;
+  if (in) {
+    v = members_old[i];
+    R = mroot_old[i];
+  }
+  if (!live || !in || R >= n || v >= n) return;
+  const uint2 t = nd[R];
+  if (t.x == gr || t.x >= n) return;
+  const uint32_t p = off[t.x] + i + t.y;
+  if (p >= e_new || p >= n) return;
+  members_new[p] = v;
+  mroot_new[p] = t.x;
+  parent[v] = t.x;
 }
 
 // Segmented XOR-reduce + sample (range_merge of column `round` + the deterministic bucket over
@@ -1988,6 +2144,8 @@ void launch_round_state_init(const RoundBufs& rb, uint32_t n, hipStream_t s) {
   launch_fill_u32(rb.totals + kStGiantFirst, kNoRoot, 1, s);
   launch_fill_u32(rb.members[0], 0, n, s);
   launch_fill_u32(rb.members[1], 0, n, s);
+  launch_fill_u32(rb.mroot[0], 0, n, s);
+  launch_fill_u32(rb.mroot[1], 0, n, s);
 }
 
 void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, const DsuDev& d, const RoundBufs& rb,
@@ -2001,7 +2159,7 @@ void launch_sample_round0(int gbytes, const unsigned char* slab, const Geo& g, c
     hipLaunchKernelGGL(k_sample_round0<uint64_t>, dim3(grid), dim3(256), 0, s, slab, g, samp, d.flags, rb.det_cpy,
                        rb.hint, d);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(g.n, 256, 4096)), dim3(256), 0, s, samp, nullptr, d, nullptr, 0u,
-                     nullptr, nullptr, d.mod, rb.hint);
+                     nullptr, d.mod, rb.hint);
 }
 
 // Host bound on each level's entry count (the kernels compute the exact counts on the device):
@@ -2016,27 +2174,37 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
   const uint32_t* prev = d.mod + (round - 1);
   uint32_t* cur = d.mod + round;
   uint32_t* st = rb.totals;
-  const uint32_t* act_list = rb.members[(round - 1) & 1];  // read only in active-list mode
-  uint32_t* members = rb.members[round & 1];
-  const uint32_t grid = (n + kAggBlock - 1) / kAggBlock;  // one walked vertex per thread
-  hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, rb.cnt, st, round,
-                     act_list, prev);
-  hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, rb.cnt, n, rb.off, rb.cursor, st, round, act_list,
-                     giant_ok ? 1u : 0u, prev);
-  hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, rb.mroot, st,
-                     round, act_list, prev);
+  const uint32_t par = round & 1, opar = (round - 1) & 1;
+  uint32_t* members = rb.members[par];
+  const uint32_t grid = (n + kAggBlock - 1) / kAggBlock;  // one vertex, or one old segment, per thread
+  if (round == 1) {  // every vertex in id order
+    const uint32_t* act_list = rb.members[opar];  // not read in round 1
+    hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, rb.cnt, st, round,
+                       act_list, prev);
+    hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, rb.cnt, n, rb.off, rb.cursor, st, round,
+                       act_list, giant_ok ? 1u : 0u, prev, rb.segs[par]);
+    hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, rb.mroot[par],
+                       st, round, act_list, prev);
+  } else {  // whole runs of the previous round's member list
+    hipLaunchKernelGGL(k_seg_find, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.cnt, rb.nd, st, round,
+                       rb.segs[opar], prev);
+    hipLaunchKernelGGL(k_seg_alloc, dim3(grid), dim3(kAggBlock), 0, s, rb.cnt, n, rb.nd, rb.off, rb.cursor,
+                       rb.segs[par], st, round, rb.segs[opar], giant_ok ? 1u : 0u, prev);
+    hipLaunchKernelGGL(k_seg_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, rb.nd, rb.off, d.parent, rb.members[opar],
+                       rb.mroot[opar], members, rb.mroot[par], st, round, prev);
+  }
   // one accumulator slot per workgroup and one for the giant
   const uint32_t rgrid = reduce_blocks(n);
   if (rgrid + 1 > rb.xslots) throw std::logic_error("reduce grid exceeds its cross-workgroup accumulator slots");
   const XAcc xa{rb.xacc, rb.xcnt, rb.xslots, rb.off, rb.cursor};
   if (gbytes == 4)
-    hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot,
+    hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot[par],
                        slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
   else
-    hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot,
+    hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot[par],
                        slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
-  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.root, d, st, round,
-                     act_list, prev, cur, rb.hint);
+  hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.segs[par], d, st,
+                     round, prev, cur, rb.hint);
 }
 
 bool launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned char* slab, const Geo& g,
