@@ -1041,11 +1041,13 @@ k_apply_samples(const uint64_t* __restrict__ samp, const RoundSeg* __restrict__ 
 // Every kernel of round r first reads mod[r-1]: when round r-1 changed nothing the whole round is
 // a no-op, which lets the host queue several rounds before it looks at the flags.
 //
-// Regrouping by root: A) root of every vertex and members per root; B) each root gets a contiguous
-// segment [off, off+cnt) from one running cursor (contiguous, not in root order, which Borůvka does
-// not need); C) scatter the vertices into their segments.  Counts and cursors are aggregated per
-// 1024-thread block in an LDS table first, so a root shared by the whole block (the giant component
-// of later rounds) costs one device atomic per block instead of one per wave or per vertex.
+// Regrouping by root (round 1, every vertex): A) root of every vertex, members per root, and each
+// vertex's place in its root's segment (the value its root's counter had when the vertex was
+// counted); B) each root gets a contiguous segment [off, off+cnt) from one running cursor
+// (contiguous, not in root order, which Borůvka does not need), and the round's segment list;
+// C) every vertex to off[root] + its place.  Counts are aggregated per 1024-thread block in an LDS
+// table first, so a root shared by the whole block costs one device atomic per block instead of
+// one per wave or per vertex.
 constexpr int kAggBlock = 1024;
 constexpr uint32_t kAggTable = 2048;  // >= 2 * kAggBlock, power of two
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
@@ -1060,7 +1062,7 @@ __device__ inline uint32_t lds_table_slot(uint32_t* keys, uint32_t key) {
 }
 
 __global__ void __launch_bounds__(kAggBlock)
-k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ root, uint32_t* __restrict__ cnt,
+k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint2* __restrict__ nd, uint32_t* __restrict__ cnt,
                 uint32_t* __restrict__ st, uint32_t round, const uint32_t* __restrict__ act_list,
                 const uint32_t* __restrict__ prev_mod) {
   // The round flag is loaded together with the first reads (all in bounds whether or not the
@@ -1090,21 +1092,26 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict_
     st[kStActMode] = act ? 1u : 0u;
   }
   __syncthreads();
+  __shared__ uint32_t t_base[kAggTable];
   uint32_t r = 0;
-  if (in_walk) {
-    r = dsu_find_stable(parent, v);
-    root[v] = r;
-  }
+  if (in_walk) r = dsu_find_stable(parent, v);
   __syncthreads();
   // full path compression: every walked vertex now points at its root (no links in this kernel)
   if (in_walk) parent[v] = r;
   const bool active = in_walk && r != gcur;  // members of the giant are not counted
   // every lane inserts its root at once: LDS atomics absorb lanes that share a root, and no wave
-  // walks its distinct roots one by one
-  if (active) atomicAdd(t_cnt + lds_table_slot(t_key, r), 1u);
+  // walks its distinct roots one by one; the returned counts give each vertex its place in its
+  // root's segment
+  uint32_t my_slot = 0, my_pos = 0;
+  if (active) {
+    my_slot = lds_table_slot(t_key, r);
+    my_pos = atomicAdd(t_cnt + my_slot, 1u);
+  }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
-    if (t_key[k] != kEmptyKey) atomicAdd(cnt + t_key[k], t_cnt[k]);
+    if (t_key[k] != kEmptyKey) t_base[k] = atomicAdd(cnt + t_key[k], t_cnt[k]);
+  __syncthreads();
+  if (in_walk) nd[v] = make_uint2(r, active ? t_base[my_slot] + my_pos : 0u);
 }
 
 // Segments for the roots met in this round's walk (every non-giant root is a member of its own
@@ -1157,44 +1164,29 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
     const uint32_t o = s_base + (uint32_t)(ex & kCntMask);
     const uint32_t k = s_sbase + (uint32_t)(ex >> kSegShift);
     off[v] = o;
-    cursor[v] = o;
+    cursor[v] = o + c;  // the segment end
     if (k < n) segs[k] = RoundSeg{v, o, c, 0};
   }
 }
 
 __global__ void __launch_bounds__(kAggBlock)
-k_bv_scatter(const uint32_t* __restrict__ root, uint32_t n, uint32_t* __restrict__ cursor,
+k_bv_scatter(const uint2* __restrict__ nd, uint32_t n, const uint32_t* __restrict__ off,
              uint32_t* __restrict__ members, uint32_t* __restrict__ mroot, const uint32_t* __restrict__ st, uint32_t round,
              const uint32_t* __restrict__ act_list, const uint32_t* __restrict__ prev_mod) {
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads (see k_bv_find_count)
   const uint32_t gr = st[kStGiantRound];
+  const uint32_t e = st[kStSeg + (round & 1)];
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
   const uint32_t walk = walk_count(st, round, n);
   const bool in_walk = i < walk;
   const uint32_t v = in_walk ? walk_vertex(act_list, round, i) : 0;
-  const uint32_t r = in_walk ? root[v] : 0;
-  if (!live || blockIdx.x * kAggBlock >= walk) return;  // whole blocks past the walk
-  __shared__ uint32_t t_key[kAggTable], t_cnt[kAggTable], t_base[kAggTable];
-  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock) {
-    t_key[k] = kEmptyKey;
-    t_cnt[k] = 0;
-  }
-  __syncthreads();
-  const bool active = in_walk && r != gr;  // the giant supernode's members are not regrouped
-  uint32_t my_slot = 0, my_pos = 0;
-  if (active) {  // every lane at once; the returned count is the lane's position in the block's share
-    my_slot = lds_table_slot(t_key, r);
-    my_pos = atomicAdd(t_cnt + my_slot, 1u);
-  }
-  __syncthreads();
-  for (uint32_t k = threadIdx.x; k < kAggTable; k += kAggBlock)
-    if (t_key[k] != kEmptyKey) t_base[k] = atomicAdd(cursor + t_key[k], t_cnt[k]);
-  __syncthreads();
-  if (active) {  // the root beside the member, so the reduce reads both in list order
-    const uint32_t p = t_base[my_slot] + my_pos;
-    members[p] = v;
-    mroot[p] = r;
-  }
+  const uint2 t = in_walk && v < n ? nd[v] : make_uint2(0u, 0u);
+  if (!live || !in_walk || v >= n) return;
+  if (t.x == gr || t.x >= n) return;  // the giant supernode's members are not regrouped
+  const uint32_t p = off[t.x] + t.y;
+  if (p >= e || p >= n) return;
+  members[p] = v;  // the root beside the member, so the reduce reads both in list order
+  mroot[p] = t.x;
 }
 
 // ---- regrouping by segments (rounds >= 2) ------------------------------------------------------
@@ -2179,11 +2171,11 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
   const uint32_t grid = (n + kAggBlock - 1) / kAggBlock;  // one vertex, or one old segment, per thread
   if (round == 1) {  // every vertex in id order
     const uint32_t* act_list = rb.members[opar];  // not read in round 1
-    hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.root, rb.cnt, st, round,
+    hipLaunchKernelGGL(k_bv_find_count, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.nd, rb.cnt, st, round,
                        act_list, prev);
     hipLaunchKernelGGL(k_bv_alloc, dim3(grid), dim3(kAggBlock), 0, s, rb.cnt, n, rb.off, rb.cursor, st, round,
                        act_list, giant_ok ? 1u : 0u, prev, rb.segs[par]);
-    hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.root, n, rb.cursor, members, rb.mroot[par],
+    hipLaunchKernelGGL(k_bv_scatter, dim3(grid), dim3(kAggBlock), 0, s, rb.nd, n, rb.off, members, rb.mroot[par],
                        st, round, act_list, prev);
   } else {  // whole runs of the previous round's member list
     hipLaunchKernelGGL(k_seg_find, dim3(grid), dim3(kAggBlock), 0, s, d.parent, n, rb.cnt, rb.nd, st, round,

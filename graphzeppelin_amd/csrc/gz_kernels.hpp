@@ -96,15 +96,15 @@ struct RoundSeg {
   uint32_t root, start, len, pad;
 };
 struct RoundBufs {
-  uint32_t* root;        // n      root of every vertex (round 1)
   uint32_t* cnt;         // n      members per root (zero between rounds)
   uint32_t* off;         // n      segment start per root
   uint32_t* cursor;      // n      scatter cursor per root (the segment end after the scatter)
   uint32_t* members[2];  // n each vertices grouped by root, by round parity (also the next active list)
   uint32_t* mroot[2];    // n each the root of members[parity][p] at position p
   RoundSeg* segs[2];     // n each the round's supernodes outside the giant, by round parity
-  uint2* nd;             // n      rounds >= 2, per previous-round root: new root, and the shift
-                         //        from its old run to its place in the new member list
+  uint2* nd;             // n      round 1, per vertex: its root and its place in the root's segment;
+                         //        rounds >= 2, per previous-round root: its new root, and the shift
+                         //        from its old run to its place in the new root's segment
   uint32_t* totals;      // 8      round state words (entry counts, giant roots, walk mode)
   uint64_t* xacc;        // xslots * 128: cross-workgroup supernode sums, zero between rounds
   uint32_t* xcnt;        // xslots arrival counters, zero between rounds

@@ -176,12 +176,12 @@ uint64_t GzAlg::footprint(uint32_t n, const gz_alg_config& c) {
   const uint64_t part = n <= (1u << 24) ? cap * 16 /*records*/ + 8 * 1048577ull + (1 << 16) : 0;
   const uint64_t ingest = cap * 8 /*pending*/ + cap * 8 /*dst_arr*/ + cap /*merged*/ + part +
                           (2 * cap / t + n + 1) * sizeof(Tile) + (uint64_t)n * 20;
-  // per vertex: root, cnt, off, cursor, 2 member lists, 2 member-root lists, parent, labels (4 B each),
+  // per vertex: cnt, off, cursor, 2 member lists, 2 member-root lists, parent, labels (4 B each),
   // sample record, forest edge, new root and shift (8 each), 2 segment lists (16 each), spanning-forest
   // hash set (16), deterministic-bucket copy, degree class (1); the reduce's cross-workgroup
   // accumulators (1 KiB + 4 B per 1024 vertices, and one more slot)
   const uint64_t rounds =
-      (uint64_t)n * (10 * 4 + 8 + 8 + 8 + 2 * 16 + 16 + 8 + gbytes + 1) + ((uint64_t)n / 512 + 3) * 1028 + 8192;
+      (uint64_t)n * (9 * 4 + 8 + 8 + 8 + 2 * 16 + 16 + 8 + gbytes + 1) + ((uint64_t)n / 512 + 3) * 1028 + 8192;
   return slab + ingest + rounds;
 }
 
@@ -238,7 +238,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   launch_dsu_reset(dsu, st);
 
   // Borůvka round buffers
-  r_root.alloc(n); r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
+  r_cnt.alloc(n); r_off.alloc(n); r_cursor.alloc(n); r_samp.alloc(n);
   r_members[0].alloc(n); r_members[1].alloc(n); r_mroot[0].alloc(n); r_mroot[1].alloc(n);
   r_segs[0].alloc(n); r_segs[1].alloc(n); r_nd.alloc(n);
   r_totals.alloc(16);
@@ -252,7 +252,7 @@ GzAlg::GzAlg(uint32_t n_, uint64_t seed_, const gz_alg_config& c) : n(n_), seed(
   r_xcnt.alloc(xslots);
   launch_fill_u64(r_xacc.p, 0, xslots * 128, st);
   launch_fill_u32(r_xcnt.p, 0, xslots, st);
-  rb = RoundBufs{r_root.p, r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p},
+  rb = RoundBufs{r_cnt.p, r_off.p, r_cursor.p, {r_members[0].p, r_members[1].p},
                  {r_mroot[0].p, r_mroot[1].p}, {r_segs[0].p, r_segs[1].p}, r_nd.p, r_totals.p,
                  r_xacc.p, r_xcnt.p, (uint32_t)xslots, r_samp.p, r_stats.p, r_det.p, r_hint.p};
   launch_fill_u32(r_cnt.p, 0, n, st);

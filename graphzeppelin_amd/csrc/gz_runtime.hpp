@@ -168,7 +168,7 @@ class GzAlg {
   DevBuf<uint8_t> qout;
   DsuDev dsu{};
   // Borůvka round buffers
-  DevBuf<uint32_t> r_root, r_cnt, r_off, r_cursor, r_members[2], r_mroot[2], r_totals, r_stats, mod;
+  DevBuf<uint32_t> r_cnt, r_off, r_cursor, r_members[2], r_mroot[2], r_totals, r_stats, mod;
   DevBuf<RoundSeg> r_segs[2];
   DevBuf<uint2> r_nd;
   DevBuf<uint64_t> r_xacc, r_samp;
