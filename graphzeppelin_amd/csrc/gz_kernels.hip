@@ -936,7 +936,7 @@ k_sample_round0(const unsigned char* __restrict__ slab, Geo g, uint64_t* __restr
 constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot, no giant)
 
 // Round state words (RoundBufs::totals).  A query starts with kStGiantFirst = kNoRoot.
-//   kStSeg + (r & 1)  entries regrouped in round r (the level-1 entry count, and the length of the
+//   kStSeg + 2(r & 1) entries regrouped in round r (the level-1 entry count, and the length of the
 //                     active list of round r + 1)
 //   kStGiantRound     root of this round's giant supernode, or kNoRoot
 //   kStGiantFirst     the root elected giant first in this query (any member identifies the giant,
@@ -950,9 +950,11 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // no supernode (unused partial slot,
 //                     vertices of (count << 32 | root); the next round excludes that supernode.
 //                     The complement sum needs no majority -- any supernode's sum is the XOR of the
 //                     vertices outside it -- so a large one can be left out a round early.
-//   kStNseg + (r & 1) supernodes outside the giant in round r (entries of RoundBufs::segs[r & 1])
-constexpr uint32_t kStSeg = 0, kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStBest = 6, kStNseg = 8;
-constexpr uint32_t kStWords = 10;
+//   kStSeg + 2(r & 1) + 1  supernodes outside the giant in round r (entries of RoundBufs::segs[r & 1])
+// The two counts of one round parity share a 64-bit word, so a block takes its share of both with
+// one atomic: entries at kStSeg + 2 * (r & 1), supernodes at kStSeg + 2 * (r & 1) + 1.
+constexpr uint32_t kStGiantRound = 2, kStGiantFirst = 3, kStActMode = 4, kStBest = 6, kStSeg = 8;
+constexpr uint32_t kStWords = 12;
 constexpr uint32_t kCandidateDiv = 32;
 constexpr uint32_t kMaxStatRounds = 64;
 
@@ -992,7 +994,7 @@ __global__ void k_dsu_reset(uint32_t* parent, uint32_t n, uint64_t* sf_hash, uin
 }
 
 __device__ inline uint32_t walk_count(const uint32_t* st, uint32_t round, uint32_t n) {
-  return round >= 2 ? st[kStSeg + ((round - 1) & 1)] : n;
+  return round >= 2 ? st[kStSeg + 2 * ((round - 1) & 1)] : n;
 }
 __device__ inline uint32_t walk_vertex(const uint32_t* act_list, uint32_t round, uint32_t i) {
   return round >= 2 ? act_list[i] : i;
@@ -1025,7 +1027,7 @@ k_apply_samples(const uint64_t* __restrict__ samp, const RoundSeg* __restrict__ 
   }
   const bool act = st && st[kStActMode] != 0;
   const uint32_t gr = act ? st[kStGiantRound] : kNoRoot;
-  const uint32_t count = st ? st[kStNseg + (round & 1)] : d.n;
+  const uint32_t count = st ? st[kStSeg + 2 * (round & 1) + 1] : d.n;
   if (!live) return;
   bool modified = false;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -1085,8 +1087,8 @@ k_bv_find_count(uint32_t* __restrict__ parent, uint32_t n, uint2* __restrict__ n
   const bool act = gfirst != kNoRoot;
   const uint32_t gcur = act ? dsu_find_stable(parent, gfirst) : kNoRoot;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st[kStSeg + (round & 1)] = 0;
-    st[kStNseg + (round & 1)] = 0;
+    st[kStSeg + 2 * (round & 1)] = 0;
+    st[kStSeg + 2 * (round & 1) + 1] = 0;
     st[kStGiantRound] = gcur;
     st[kStGiantFirst] = gfirst;
     st[kStActMode] = act ? 1u : 0u;
@@ -1155,8 +1157,12 @@ k_bv_alloc(uint32_t* __restrict__ cnt, uint32_t n, uint32_t* __restrict__ off, u
   }
   constexpr uint64_t kCntMask = ((uint64_t)1 << kSegShift) - 1;
   if (threadIdx.x == 0) {
-    s_base = (tot & kCntMask) ? atomicAdd(st + kStSeg + (round & 1), (uint32_t)(tot & kCntMask)) : 0;
-    s_sbase = (tot >> kSegShift) ? atomicAdd(st + kStNseg + (round & 1), (uint32_t)(tot >> kSegShift)) : 0;
+    // entries and supernodes of the block in one 64-bit atomic (neither sum can reach 2^32)
+    const uint64_t add = ((tot >> kSegShift) << 32) | (tot & kCntMask);
+    const uint64_t old =
+        add ? atomicAdd(reinterpret_cast<unsigned long long*>(st + kStSeg + 2 * (round & 1)), (unsigned long long)add) : 0;
+    s_base = (uint32_t)old;
+    s_sbase = (uint32_t)(old >> 32);
   }
   __syncthreads();
   if (c) {
@@ -1175,7 +1181,7 @@ k_bv_scatter(const uint2* __restrict__ nd, uint32_t n, const uint32_t* __restric
              const uint32_t* __restrict__ act_list, const uint32_t* __restrict__ prev_mod) {
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads (see k_bv_find_count)
   const uint32_t gr = st[kStGiantRound];
-  const uint32_t e = st[kStSeg + (round & 1)];
+  const uint32_t e = st[kStSeg + 2 * (round & 1)];
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
   const uint32_t walk = walk_count(st, round, n);
   const bool in_walk = i < walk;
@@ -1210,7 +1216,7 @@ k_seg_find(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ cnt
   const uint32_t live = ld_acq(prev_mod);
   uint32_t gfirst = st[kStGiantFirst];
   const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
-  const uint32_t nseg = st[kStNseg + ((round - 1) & 1)];
+  const uint32_t nseg = st[kStSeg + 2 * ((round - 1) & 1) + 1];
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
   const bool in = i < nseg;
   RoundSeg sg{0, 0, 0, 0};
@@ -1226,8 +1232,8 @@ k_seg_find(uint32_t* __restrict__ parent, uint32_t n, uint32_t* __restrict__ cnt
   const bool act = gfirst != kNoRoot;
   const uint32_t gcur = act ? dsu_find_stable(parent, gfirst) : kNoRoot;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st[kStSeg + (round & 1)] = 0;
-    st[kStNseg + (round & 1)] = 0;
+    st[kStSeg + 2 * (round & 1)] = 0;
+    st[kStSeg + 2 * (round & 1) + 1] = 0;
     st[kStGiantRound] = gcur;
     st[kStGiantFirst] = gfirst;
     st[kStActMode] = act ? 1u : 0u;
@@ -1263,7 +1269,7 @@ k_seg_alloc(uint32_t* __restrict__ cnt, uint32_t n, const uint2* __restrict__ nd
   __shared__ uint64_t wsum[kAggBlock / kWave];
   __shared__ uint32_t s_base, s_sbase;
   const bool act = st[kStActMode] != 0;
-  const uint32_t nseg = st[kStNseg + ((round - 1) & 1)];
+  const uint32_t nseg = st[kStSeg + 2 * ((round - 1) & 1) + 1];
   const uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
   const uint32_t i = blockIdx.x * kAggBlock + threadIdx.x;
   const bool in = i < nseg;
@@ -1291,8 +1297,12 @@ k_seg_alloc(uint32_t* __restrict__ cnt, uint32_t n, const uint2* __restrict__ nd
   }
   constexpr uint64_t kCntMask = ((uint64_t)1 << kSegShift) - 1;
   if (threadIdx.x == 0) {
-    s_base = (tot & kCntMask) ? atomicAdd(st + kStSeg + (round & 1), (uint32_t)(tot & kCntMask)) : 0;
-    s_sbase = (tot >> kSegShift) ? atomicAdd(st + kStNseg + (round & 1), (uint32_t)(tot >> kSegShift)) : 0;
+    // entries and supernodes of the block in one 64-bit atomic (neither sum can reach 2^32)
+    const uint64_t add = ((tot >> kSegShift) << 32) | (tot & kCntMask);
+    const uint64_t old =
+        add ? atomicAdd(reinterpret_cast<unsigned long long*>(st + kStSeg + 2 * (round & 1)), (unsigned long long)add) : 0;
+    s_base = (uint32_t)old;
+    s_sbase = (uint32_t)(old >> 32);
   }
   __syncthreads();
   if (c) {
@@ -1314,8 +1324,8 @@ k_seg_copy(uint32_t n, const uint2* __restrict__ nd, const uint32_t* __restrict_
            uint32_t round, const uint32_t* __restrict__ prev_mod) {
   const uint32_t live = ld_acq(prev_mod);
   const uint32_t gr = st[kStGiantRound];
-  const uint32_t e_old = st[kStSeg + ((round - 1) & 1)];
-  const uint32_t e_new = st[kStSeg + (round & 1)];
+  const uint32_t e_old = st[kStSeg + 2 * ((round - 1) & 1)];
+  const uint32_t e_new = st[kStSeg + 2 * (round & 1)];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < e_old && i < n;
   uint32_t v = 0, R = 0;
@@ -1539,7 +1549,7 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
   GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
-  const uint32_t e1 = st[kStSeg + (round & 1)];
+  const uint32_t e1 = st[kStSeg + 2 * (round & 1)];
   if (!live) return;
   if (rstats && blockIdx.x == 0 && threadIdx.x == 0 && round < kMaxStatRounds) {
     rstats[2 * round] = e1;  // diagnostics: vertices regrouped and the giant's root in this round
@@ -1704,7 +1714,7 @@ k_bv_finish(DsuDev d, uint32_t* __restrict__ st, const uint32_t* __restrict__ ac
     uint32_t gfirst = st[kStGiantFirst];
     const uint64_t best = *reinterpret_cast<const uint64_t*>(st + kStBest);
     if (gfirst == kNoRoot && best) gfirst = (uint32_t)best;  // as k_bv_find_count takes the candidate
-    const uint32_t walk = st[kStSeg + ((r0 - 1) & 1)];
+    const uint32_t walk = st[kStSeg + 2 * ((r0 - 1) & 1)];
     s_take = (live && gfirst != kNoRoot && walk <= kFinishMax) ? 1u : 0u;
     s_gfirst = gfirst;
     s_walk = walk;
