@@ -378,7 +378,7 @@ def main() -> int:
 def multi_tenant_leg(gz, gen, torch) -> dict:
     """32 R-MAT streams (n log-uniform in [2^12, 2^20], 8 n log2 n updates, 10% deletes) served as
     one mix: everything arriving at once (aggregate throughput), then every stream arriving at a
-    steady rate, together half of that throughput (latency under load)."""
+    steady rate, together a quarter and half of that throughput (latency under load)."""
     T, seed = 32, 5
     t0 = time.time()
     tenants = gen.multi_tenant_workload(T, seed)
@@ -403,8 +403,12 @@ def multi_tenant_leg(gz, gen, torch) -> dict:
 
     sat = run(None)
     out["saturating"] = sat
+    # steady arrivals make smaller flushes than everything arriving at once, so a fraction of the
+    # saturating rate need not be sustainable: each load point reports what it achieved
+    out["quarter_load"] = run(0.25 * sat["value"])
     out["half_load"] = run(0.5 * sat["value"])
-    log(f"multi-tenant: {sat['value'] / 1e6:.1f} M updates/s saturating; at half load p50/p99 "
+    log(f"multi-tenant: {sat['value'] / 1e6:.1f} M updates/s saturating; p50/p99 at a quarter of that "
+        f"{out['quarter_load']['p50_query_ms']:.1f}/{out['quarter_load']['p99_query_ms']:.1f} ms, at half "
         f"{out['half_load']['p50_query_ms']:.1f}/{out['half_load']['p99_query_ms']:.1f} ms")
     del tenants
     torch.cuda.empty_cache()

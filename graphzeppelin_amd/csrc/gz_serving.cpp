@@ -299,15 +299,23 @@ void Server::run_trace(const gz_request* reqs, size_t nreq, const uint32_t* cons
     std::unique_lock<std::mutex> lk(m);
     while (true) {
       if (!err.empty() || finished == admit_order.size()) return;
-      // pick an admitted, idle tenant whose next request has arrived; else wait
+      // pick the admitted, idle tenant whose next request arrived first (first come, first served
+      // across tenants, so no tenant waits behind others that keep arriving); else wait
       size_t pick = T;
-      double next_arrival = 1e300;
+      double next_arrival = 1e300, oldest = 1e300;
+      const double now = now_s();
       for (size_t t : admit_order) {
         TState& s = ts[t];
         if (s.id < 0 || s.busy || s.done || s.todo.empty()) continue;
         const double arr = reqs[s.todo.front()].arrival_s;
-        if (arr <= now_s()) { pick = t; break; }
-        next_arrival = std::min(next_arrival, arr);
+        if (arr <= now) {
+          if (arr < oldest) {
+            oldest = arr;
+            pick = t;
+          }
+        } else {
+          next_arrival = std::min(next_arrival, arr);
+        }
       }
       if (pick == T) {
         size_t active = 0;
