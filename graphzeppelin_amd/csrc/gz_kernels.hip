@@ -302,7 +302,7 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
 // grid-stride (no shared counter), so a persistent grid can cover a tile count that only the
 // device knows.
 // ================================================================================================
-constexpr uint32_t kLdsDepth = 8, kLdsPitch = 9;
+constexpr uint32_t kLdsDepth = 9, kLdsPitch = 9;  // depths 0..8 per column; 9 is odd
 
 template <class G, bool RAW>
 __global__ void __launch_bounds__(kWave, 8)  // 8 waves per SIMD: 32 tiles in flight per CU
@@ -323,7 +323,8 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
 
   for (uint32_t i = lane; i < C; i += kWave) s_bf[i] = col_bf[i];
   // The delta is zeroed once here; afterwards the write-back clears every entry it reads.  Entries
-  // at the pitch padding (d == kLdsDepth) or at depth >= D are never written: offset 0.
+  // at depth >= D (or at a pitch padding, if the pitch exceeds kLdsDepth) are never written:
+  // offset 0.
   for (uint32_t i = lane; i < lds_n; i += kWave) {
     s_a[i] = 0;
     s_g[i] = 0;
