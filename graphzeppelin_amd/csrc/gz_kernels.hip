@@ -341,40 +341,45 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
   // Updates are taken in chunks of 64: lane l loads update u0 + l once and hashes its index and
   // checksum once into a 16-byte LDS slot; the chunk's (update, column) pairs then read their
   // update's slot with one LDS load, so each pair costs one column hash and no memory round trip.
-  // The next tile's descriptor and each lane's first index are loaded before the current tile's
-  // write-back.
-  // The prefetched word is the raw destination (or raw index): the pairing is formed when the
-  // tile starts, so waiting for the load does not precede the write-back.
+  // The next tile's descriptor and each lane's first destination (or raw index) are loaded
+  // before the current tile's write-back, and that chunk is hashed between the write-back's
+  // record loads and its stores, where the wave waits for those loads anyway.  No tile then
+  // starts with a wait, which would also wait for the previous tile's stores (the vector memory
+  // counter covers stores too).
   auto load_raw = [&](const Tile& tt, uint32_t u) -> uint64_t {
     return RAW ? raw_idx[tt.begin + u] : (uint64_t)dsts[tt.begin + u];
   };
   auto to_idx = [&](const Tile& tt, uint64_t w) -> uint64_t {
     return RAW ? w : concat_pairing(tt.vertex, (uint32_t)w);
   };
+  // per-lane XOR of the tile's indices and checksums: the deterministic bucket's delta
+  uint64_t det_a = 0;
+  G det_g = 0;
+  auto hash_chunk = [&](const Tile& tt, uint32_t u0, uint64_t raw) {
+    const uint32_t m = tt.count - u0 < kWave ? tt.count - u0 : kWave;
+    uint64_t my_pre = 0;
+    G my_chk = 0;
+    if (lane < m) {
+      const uint64_t idx = to_idx(tt, raw);
+      my_pre = xxh3_prep(idx);
+      my_chk = (G)xxh3_mix(my_pre, g.chk_bf);
+      det_a ^= idx;
+      det_g ^= my_chk;
+    }
+    s_pc[lane] = make_ulonglong2(my_pre, (uint64_t)my_chk);  // the wave's LDS ops stay in order
+  };
   uint32_t t = blockIdx.x;
   Tile tl{0, 0, 0, 0};
-  uint64_t raw0 = 0;
   if (t < ntiles) {
     tl = tiles[t];
-    if (lane < tl.count) raw0 = load_raw(tl, lane);
+    hash_chunk(tl, 0, lane < tl.count ? load_raw(tl, lane) : 0);
   }
   while (t < ntiles) {
     unsigned char* row = slab + (size_t)tl.vertex * g.row;
 
-    uint64_t det_a = 0;
-    G det_g = 0;
     for (uint32_t u0 = 0; u0 < tl.count; u0 += kWave) {
       const uint32_t m = tl.count - u0 < kWave ? tl.count - u0 : kWave;
-      uint64_t my_pre = 0;
-      G my_chk = 0;
-      if (lane < m) {
-        const uint64_t idx = to_idx(tl, u0 == 0 ? raw0 : load_raw(tl, u0 + lane));
-        my_pre = xxh3_prep(idx);
-        my_chk = (G)xxh3_mix(my_pre, g.chk_bf);
-        det_a ^= idx;
-        det_g ^= my_chk;
-      }
-      s_pc[lane] = make_ulonglong2(my_pre, (uint64_t)my_chk);  // the wave's LDS ops stay in order
+      if (u0 != 0) hash_chunk(tl, u0, lane < m ? load_raw(tl, u0 + lane) : 0);
       const uint32_t np = m * C;
       const uint32_t iters = (np + kWave - 1) / kWave;  // uniform over the wave
       uint32_t q = i0, c = c0;
@@ -404,11 +409,15 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
         }
       }
     }
-    det_a = wave_xor_dpp_u64(det_a);
-    det_g = sizeof(G) == 4 ? (G)wave_xor_dpp_u32((uint32_t)det_g) : (G)wave_xor_dpp_u64((uint64_t)det_g);
-    if (lane == 0 && (det_a | (uint64_t)det_g)) {
-      s_a[det_slot] ^= det_a;
-      s_g[det_slot] ^= det_g;
+    {
+      const uint64_t da = wave_xor_dpp_u64(det_a);
+      const G dg = sizeof(G) == 4 ? (G)wave_xor_dpp_u32((uint32_t)det_g) : (G)wave_xor_dpp_u64((uint64_t)det_g);
+      if (lane == 0 && (da | (uint64_t)dg)) {
+        s_a[det_slot] ^= da;
+        s_g[det_slot] ^= dg;
+      }
+      det_a = 0;
+      det_g = 0;
     }
     __syncthreads();
 
@@ -441,17 +450,25 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
           }
         }
       }
+      if (!tl.split) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if ((da[k] | (uint64_t)dg[k]) != 0) ld_rec<G>(row + bo[k], &ga[k], &gg[k]);
+      }
+      // The next tile's first chunk, hashed while this tile's record loads are in flight.  The
+      // empty asm keeps the compiler from hoisting the hash above the loads (it would wait there
+      // for the prefetch); the record loads leave ga/gg unset for entries they skip, so no zero
+      // fill of those registers waits on earlier memory operations either.
+      if (e0 == 0 && tn < ntiles) {
+        uint32_t lo = (uint32_t)raw0_next, hi = (uint32_t)(raw0_next >> 32);
+        asm volatile("" : "+v"(lo), "+v"(hi)::"memory");
+        hash_chunk(tl_next, 0, ((uint64_t)hi << 32) | lo);
+      }
       if (tl.split) {
 #pragma unroll
         for (int k = 0; k < 4; ++k)
           if ((da[k] | (uint64_t)dg[k]) != 0) xor_rec_atomic<G>(row + bo[k], da[k], dg[k]);
       } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          ga[k] = 0;
-          gg[k] = 0;
-          if ((da[k] | (uint64_t)dg[k]) != 0) ld_rec<G>(row + bo[k], &ga[k], &gg[k]);
-        }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
           if ((da[k] | (uint64_t)dg[k]) != 0) st_rec<G>(row + bo[k], ga[k] ^ da[k], (G)(gg[k] ^ dg[k]));
@@ -460,7 +477,6 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     __syncthreads();
     t = tn;
     tl = tl_next;
-    raw0 = raw0_next;
   }
 }
 
