@@ -171,6 +171,7 @@ _SIGS = {
     "gz_alg_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_alg_set_query_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_alg_set_round_finish": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "gz_alg_set_deep_skip": (ctypes.c_int, [_vp, ctypes.c_int]),
     "gz_autotune": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, _sz, _vp, _vp]),
     "gz_alg_last_query_finish_round": (ctypes.c_uint32, [_vp]),
     "gz_alg_last_query_round_sizes": (_sz, [_vp, _vp, _vp, _sz]),
@@ -579,6 +580,10 @@ class CCSketchAlg:
     def set_round_finish(self, enable: bool = True):
         """Run the last Borůvka rounds in one single-workgroup launch (default) or as separate launches."""
         _check(lib().gz_alg_set_round_finish(self._h, 1 if enable else 0))
+
+    def set_deep_skip(self, enable: bool = True):
+        """Let Borůvka reduces skip deep records whose column deep flag is clear (default), or read them all."""
+        _check(lib().gz_alg_set_deep_skip(self._h, 1 if enable else 0))
 
     @property
     def last_query_finish_round(self) -> int:

@@ -78,6 +78,10 @@ __device__ inline void gbl_xor(T* p, T v) {
   __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <class T>
+__device__ inline void gbl_or(T* p, T v) {
+  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
 __device__ inline T ld_acq(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -102,6 +106,17 @@ __device__ inline uint32_t rec_col_off(const Geo& g, uint32_t c, uint32_t d) {
   return d < g.low ? (c >> 1) * 128u + (c & 1u) * 64u + d * RB : g.ovf + c * g.opitch + (d - g.low) * RB;
 }
 __device__ inline uint32_t det_col_off(const Geo& g) { return (g.C >> 1) * 128u + (g.C & 1u) * 64u; }
+// Deep flags (rows of 12-byte records only): the word after a column's at most five low records,
+// byte 60 of its half-line, becomes 1 once any record of the column that ends past the first line
+// of its overflow run (depth >= low + 10) has been written nonzero: every writer of such a record
+// (the update kernel's deep path, k_packed) stores 1 there, and only a reset of the row clears it.
+// A Borůvka reduce reads those records only for members whose flag is set; for the others they are
+// zero.  An update reaches such a depth once in 2^15 (update, column) pairs, so most vertices never
+// set their flags, and the reduce reads one overflow line of the column instead of three.
+template <class G>
+constexpr bool kDeepFlags = sizeof(G) == 4;
+__device__ inline uint32_t deep_flag_off(uint32_t c) { return (c >> 1) * 128u + (c & 1u) * 64u + 60u; }
+__device__ inline bool rec_is_deep(const Geo& g, uint32_t d) { return d >= g.low && (d - g.low) * 12u + 11u >= 128u; }
 template <class G>
 __device__ inline size_t rec_off(const Geo& g, uint32_t v, uint32_t b) {
   if (b == g.nb - 1) return (size_t)v * g.row + det_col_off(g);
@@ -296,15 +311,21 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
 // (update, column) pairs; the tile's delta for depths 0..kLdsDepth-1 of every column lives in LDS
 // (column pitch kLdsPitch = 9, odd, so lanes on consecutive columns hit different banks), and the
 // rare deeper contributions (one pair in 2^kLdsDepth) are XORed straight into the vertex's records
-// with device-scope atomics.  The deterministic bucket is kept in registers and reduced once per
-// tile.  The write-back is a read-xor-write of the touched records, or device-scope atomic XOR
-// when the vertex is split across tiles of the same launch.  Workgroups walk the tiles
+// with device-scope atomics (setting the column's deep flag when the record lies past the first
+// line of the column's overflow run).  The deterministic bucket is kept in registers and reduced
+// once per tile.  The write-back is a read-xor-write of the touched records, or device-scope
+// atomic XOR when the vertex is split across tiles of the same launch.  Workgroups walk the tiles
 // grid-stride (no shared counter), so a persistent grid can cover a tile count that only the
 // device knows.
 // ================================================================================================
 constexpr uint32_t kLdsDepth = 9, kLdsPitch = 9;  // depths 0..8 per column; 9 is odd
+// s_waitcnt immediate for vmcnt(0), with expcnt and lgkmcnt at their maxima (not waited on)
+constexpr unsigned kWaitVmcnt0 = 0x0F70;
 
-template <class G, bool RAW>
+// WIDE: more than 28 columns, so the write-back takes more than one pass over the LDS delta (a
+// separate instance keeps the one-pass form free of the loop, whose merged paths would make the
+// compiler wait on this tile's stores at the next tile's first use of the prefetch register).
+template <class G, bool RAW, bool WIDE>
 __global__ void __launch_bounds__(kWave, 8)  // 8 waves per SIMD: 32 tiles in flight per CU
 k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntiles_dev, uint32_t ntiles_host,
                const uint32_t* __restrict__ dsts, const uint64_t* __restrict__ raw_idx,
@@ -398,6 +419,8 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
               lds_xor(s_g + li, chk);
             } else {
               xor_rec_atomic<G>(row + rec_col_off<G>(g, c, depth), idx, chk);
+              if (kDeepFlags<G> && rec_is_deep(g, depth))  // the flag only ever goes from 0 to 1
+                st_rlx(reinterpret_cast<uint32_t*>(row + deep_flag_off(c)), 1u);
             }
           }
         }
@@ -422,6 +445,9 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     __syncthreads();
 
     const uint32_t tn = t + gridDim.x;
+    // The previous tile's stores complete before this tile's loads are issued (measured: letting
+    // them overlap is slower).
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     Tile tl_next{0, 0, 0, 0};
     uint64_t raw0_next = 0;
     if (tn < ntiles) {
@@ -430,7 +456,7 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
     }
 
     // write-back: read-xor-write of the touched records, four LDS entries per lane in flight
-    for (uint32_t e0 = 0; e0 < lds_n; e0 += 4 * kWave) {
+    auto write_back = [&](uint32_t e0, bool first) {
       uint64_t da[4], ga[4];
       G dg[4], gg[4];
       uint32_t bo[4];
@@ -456,13 +482,15 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
           if ((da[k] | (uint64_t)dg[k]) != 0) ld_rec<G>(row + bo[k], &ga[k], &gg[k]);
       }
       // The next tile's first chunk, hashed while this tile's record loads are in flight.  The
-      // empty asm keeps the compiler from hoisting the hash above the loads (it would wait there
-      // for the prefetch); the record loads leave ga/gg unset for entries they skip, so no zero
-      // fill of those registers waits on earlier memory operations either.
-      if (e0 == 0 && tn < ntiles) {
+      // empty asm, on every path of the first pass, keeps the compiler from hoisting the hash
+      // above the loads (it would wait there for the prefetch) and leaves no prefetch pending
+      // into the next tile, where a reuse of its register would wait on all of this tile's
+      // stores.  The record loads leave ga/gg unset for entries they skip, so no zero fill of
+      // those registers waits on earlier memory operations either.
+      if (first) {
         uint32_t lo = (uint32_t)raw0_next, hi = (uint32_t)(raw0_next >> 32);
         asm volatile("" : "+v"(lo), "+v"(hi)::"memory");
-        hash_chunk(tl_next, 0, ((uint64_t)hi << 32) | lo);
+        if (tn < ntiles) hash_chunk(tl_next, 0, ((uint64_t)hi << 32) | lo);
       }
       if (tl.split) {
 #pragma unroll
@@ -473,6 +501,11 @@ k_update_tiles(const Tile* __restrict__ tiles, const uint32_t* __restrict__ ntil
         for (int k = 0; k < 4; ++k)
           if ((da[k] | (uint64_t)dg[k]) != 0) st_rec<G>(row + bo[k], ga[k] ^ da[k], (G)(gg[k] ^ dg[k]));
       }
+    };
+    write_back(0, true);  // the delta holds at least the deterministic bucket's entry
+    for (uint32_t e0 = 4 * kWave; WIDE && e0 < lds_n; e0 += 4 * kWave) {
+      write_back(e0, false);
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // none of these passes' loads stays pending into the next tile
     }
     __syncthreads();
     t = tn;
@@ -1414,7 +1447,7 @@ struct XAcc {
 // with s_bnd: the wave holding the workgroup's first entry [blk_b0] loads that entry's segment
 // bounds into s_bnd[0..1], the one holding its last entry the end of that entry's segment into
 // s_bnd[2], in flight with the values.
-template <class G, bool FIRST, bool ROOTPOS = false>
+template <class G, bool FIRST, bool ROOTPOS = false, bool SKIP = false>
 __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_root, uint32_t next_root,
                                     const uint32_t* __restrict__ members, const uint32_t* __restrict__ vroot,
                                     const unsigned char* __restrict__ slab, const Geo& g, uint32_t round,
@@ -1422,7 +1455,7 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
                                     uint64_t* __restrict__ samp, uint32_t* __restrict__ flags, uint64_t* all_a_out,
                                     uint64_t* all_g_out, const unsigned char* __restrict__ det_cpy,
                                     const XAcc* xa, uint32_t blk_b0, uint32_t blk_b1, uint32_t* s_bnd,
-                                    const uint32_t* root_pos = nullptr) {
+                                    const uint32_t* root_pos = nullptr, bool deep_on = true) {
   const uint32_t lane = lane_id();
   uint32_t m = 0, rt = kNoRoot;
   if (lane < len) {
@@ -1433,6 +1466,12 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
       rt = in_root[p0 + lane];
     }
   }
+  // SKIP with deep_on (members only): each member's deep flag of column `round`; its deep records
+  // are read only when it is set.  The flag shares a line with the column's low records, which the
+  // batch loads below read anyway.
+  uint32_t deep = 1;
+  if (SKIP && FIRST && deep_on && lane < len)
+    deep = *reinterpret_cast<const uint32_t*>(slab + (size_t)m * g.row + deep_flag_off(round));
   const uint32_t first_root = rd_lane_u32(rt, 0), last_root = rd_lane_u32(rt, len - 1);
   const bool head_open = first_root != kNoRoot && prev_root == first_root;
   const bool tail_open = last_root != kNoRoot && next_root == last_root;
@@ -1447,6 +1486,7 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
   const bool in_col = lane < g.D;
   const uint32_t coff = in_col ? rec_col_off<G>(g, round, lane) : 0;  // depth `lane` of column `round`
   const bool loads = lane <= g.D;
+  const bool deep_lane = SKIP && deep_on && in_col && rec_is_deep(g, lane);
   bool wrote_head = false, wrote_tail = false;
 
   // Every lane stores the root (the same value), so every lane of this wave may read it back
@@ -1492,7 +1532,7 @@ __device__ inline void reduce_range(uint32_t p0, uint32_t len, uint32_t prev_roo
     for (uint32_t k = 0; k < U; ++k) {
       va[k] = 0;
       vg[k] = 0;
-      if (j + k < len && loads) {
+      if (j + k < len && loads && !(deep_lane && rd_lane_u32(deep, j + k) == 0)) {
         if (FIRST) {
           // lanes < D read the column's records, lane D the compact deterministic bucket: one
           // load instruction with per-lane addresses
@@ -1562,7 +1602,7 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
             const uint32_t* __restrict__ mroot, const unsigned char* __restrict__ slab, Geo g, uint32_t round,
             XAcc xa, uint64_t* __restrict__ samp, uint32_t* __restrict__ flags,
             const uint32_t* __restrict__ prev_mod, uint32_t* __restrict__ rstats,
-            const unsigned char* __restrict__ det_cpy, uint32_t small_walk) {
+            const unsigned char* __restrict__ det_cpy, uint32_t small_walk, uint32_t deep_skip) {
   const uint32_t live = ld_acq(prev_mod);  // checked after the first reads
   GZ_REDUCE_LDS
   const uint32_t giant = st[kStGiantRound];
@@ -1604,8 +1644,9 @@ k_bv_reduce(const uint32_t* __restrict__ st, const uint32_t* __restrict__ member
     const uint32_t prev = p0 == 0 ? kNoRoot : root_at(p0 - 1);
     const uint32_t next = p0 + len >= e1 ? kNoRoot : root_at(p0 + len);
     uint64_t wa, wg;
-    reduce_range<G, true, true>(p0, len, prev, next, members, nullptr, slab, g, round, nullptr, nullptr, hs, ts, samp,
-                                flags, &wa, &wg, det_cpy, &xa, b0, b1, s_bnd, mroot);
+    reduce_range<G, true, true, kDeepFlags<G>>(p0, len, prev, next, members, nullptr, slab, g, round, nullptr,
+                                               nullptr, hs, ts, samp, flags, &wa, &wg, det_cpy, &xa, b0, b1, s_bnd,
+                                               mroot, deep_skip != 0);
     if (giant_tot) {
       lds_xor(s_tot + lane, wa);
       lds_xor(s_tot + kWave + lane, wg);
@@ -1988,7 +2029,8 @@ __global__ void k_xor_range(unsigned char* __restrict__ dst, const unsigned char
 }
 
 // Packed reference image: record i of sketch v at byte (v * nb + i) * (8 + sizeof(G)); the slab row
-// holds the same records followed by padding.
+// holds the same records, padding and the deep flags (set here for every deep record written
+// nonzero; a stale set flag only costs the reduce a read).
 template <class G, int MODE>  // 0 = xor image into slab, 1 = overwrite slab, 2 = slab to image
 __global__ void k_packed(unsigned char* __restrict__ slab, Geo g, uint32_t v0, uint32_t nv,
                          unsigned char* __restrict__ packed) {
@@ -2004,8 +2046,14 @@ __global__ void k_packed(unsigned char* __restrict__ slab, Geo g, uint32_t v0, u
       st_rec<G>(packed + i * rec, a, gm);
     } else {
       ld_rec<G>(packed + i * rec, &pa, &pg);
-      if (MODE == 0) st_rec<G>(r, a ^ pa, (G)(gm ^ pg));
-      else st_rec<G>(r, pa, pg);
+      const uint64_t na = MODE == 0 ? a ^ pa : pa;
+      const G ng = MODE == 0 ? (G)(gm ^ pg) : pg;
+      st_rec<G>(r, na, ng);
+      if (kDeepFlags<G> && b != g.nb - 1 && (na | (uint64_t)ng) != 0) {
+        const uint32_t c = b / g.D;
+        if (rec_is_deep(g, b - c * g.D))
+          st_rlx(reinterpret_cast<uint32_t*>(slab + (size_t)(v0 + v) * g.row + deep_flag_off(c)), 1u);
+      }
     }
   }
 }
@@ -2085,20 +2133,17 @@ void launch_update_tiles(int gbytes, const Tile* tiles, const uint32_t* ntiles_d
   uint32_t grid = ntiles_dev ? grid_cap : (grid_cap && ntiles_host > grid_cap ? grid_cap : ntiles_host);
   if (grid == 0) return;
   const size_t lds = update_tiles_lds_bytes(g, gbytes);
+  const bool wide = g.C * kLdsPitch + 1 > 4 * kWave;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev, ntiles_host, dsts, raw_idx, slab, g,
+                       col_bf);
+  };
   if (gbytes == 4) {
-    if (raw_idx)
-      hipLaunchKernelGGL((k_update_tiles<uint32_t, true>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
-                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
-    else
-      hipLaunchKernelGGL((k_update_tiles<uint32_t, false>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
-                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
+    if (raw_idx) wide ? go(k_update_tiles<uint32_t, true, true>) : go(k_update_tiles<uint32_t, true, false>);
+    else wide ? go(k_update_tiles<uint32_t, false, true>) : go(k_update_tiles<uint32_t, false, false>);
   } else {
-    if (raw_idx)
-      hipLaunchKernelGGL((k_update_tiles<uint64_t, true>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
-                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
-    else
-      hipLaunchKernelGGL((k_update_tiles<uint64_t, false>), dim3(grid), dim3(kWave), lds, s, tiles, ntiles_dev,
-                         ntiles_host, dsts, raw_idx, slab, g, col_bf);
+    if (raw_idx) wide ? go(k_update_tiles<uint64_t, true, true>) : go(k_update_tiles<uint64_t, true, false>);
+    else wide ? go(k_update_tiles<uint64_t, false, true>) : go(k_update_tiles<uint64_t, false, false>);
   }
 }
 
@@ -2188,7 +2233,7 @@ uint32_t reduce_blocks(uint32_t n) { return 2 * std::max(1u, (n + kRedBlock - 1)
 constexpr uint32_t kSmallWalkDiv = 2;  // rounds regrouping at most n / kSmallWalkDiv entries use 32 per wave
 
 void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
-                  const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s) {
+                  const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s, bool deep_skip) {
   const uint32_t n = g.n;
   const uint32_t* prev = d.mod + (round - 1);
   uint32_t* cur = d.mod + round;
@@ -2218,10 +2263,12 @@ void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const G
   const XAcc xa{rb.xacc, rb.xcnt, rb.xslots, rb.off, rb.cursor};
   if (gbytes == 4)
     hipLaunchKernelGGL(k_bv_reduce<uint32_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot[par],
-                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
+                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv,
+                       deep_skip ? 1u : 0u);
   else
     hipLaunchKernelGGL(k_bv_reduce<uint64_t>, dim3(rgrid), dim3(kRedBlock), 0, s, st, members, rb.mroot[par],
-                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv);
+                       slab, g, round, xa, rb.samp, d.flags, prev, rb.rstats, rb.det_cpy, n / kSmallWalkDiv,
+                       deep_skip ? 1u : 0u);
   hipLaunchKernelGGL(k_apply_samples, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, rb.samp, rb.segs[par], d, st,
                      round, prev, cur, rb.hint);
 }

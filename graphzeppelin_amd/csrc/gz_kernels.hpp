@@ -132,9 +132,11 @@ bool launch_finish(int gbytes, uint32_t r0, uint32_t last_round, const unsigned 
 uint32_t finish_max();
 // Queue round `round` (>= 1); every kernel of it is a no-op when d.mod[round - 1] == 0.
 // giant_ok: the vertex sketches are known to XOR to zero, so a supernode holding more than half of
-// the vertices may be summed from its complement.
+// the vertices may be summed from its complement.  deep_skip: the reduce reads a member's records
+// past the first overflow line of the round's column only when its deep flag is set (rows of
+// 12-byte records); false reads them all (same result).
 void launch_round(int gbytes, uint32_t round, const unsigned char* slab, const Geo& g,
-                  const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s);
+                  const DsuDev& d, const RoundBufs& rb, bool giant_ok, hipStream_t s, bool deep_skip = true);
 void launch_labels(const DsuDev& d, uint32_t* labels, uint32_t* ncc, hipStream_t s);
 // A query group's answers in place in mapped host memory: q[0..nq) and q[nq..2nq) are the pairs;
 // out gets the kNumFlags error flags, then one byte per pair (1: connected).

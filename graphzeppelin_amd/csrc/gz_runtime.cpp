@@ -746,12 +746,12 @@ bool GzAlg::sketches_sum_to_zero() {
 void GzAlg::enqueue_query_prefix(bool giant_ok, uint32_t last) {
   launch_dsu_reset(dsu, st, &rb);  // with the root counts and the round state words
   launch_sample_round0(gb, slab.p, geo, dsu, rb, st);
-  for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
+  for (uint32_t r = 1; r <= last; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st, use_deep_skip);
   if (giant_ok && use_finish) launch_finish(gb, last + 1, geo.samples - 1, slab.p, geo, dsu, rb, st, rep.d);
 }
 
 hipGraphExec_t GzAlg::get_query_graph(bool giant_ok, uint32_t last) {
-  const uint32_t key = (last * 2 + (giant_ok ? 1 : 0)) * 2 + (use_finish ? 1 : 0);
+  const uint32_t key = ((last * 2 + (giant_ok ? 1 : 0)) * 2 + (use_finish ? 1 : 0)) * 2 + (use_deep_skip ? 1 : 0);
   hipGraphExec_t& slot = query_graphs[key];
   if (slot) return slot;
   GZ_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -847,7 +847,7 @@ bool GzAlg::boruvka(const std::function<void()>* speculate, bool gated, bool* sk
       if (upto >= last_round)
         throw GzError(GZ_ERR_OUT_OF_SAMPLES, "Borůvka needs more rounds than the sketches have samples");
       const uint32_t hi = std::min(next + 1, last_round);
-      for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st);
+      for (uint32_t r = next; r <= hi; ++r) launch_round(gb, r, slab.p, geo, dsu, rb, giant_ok, st, use_deep_skip);
       reported = giant_ok && use_finish && launch_finish(gb, hi + 1, last_round, slab.p, geo, dsu, rb, st, rep.d);
       GZ_HIP(hipGetLastError());
       fin_r0 = hi + 1;
@@ -1463,6 +1463,12 @@ int gz_alg_set_round_finish(gz_alg* a, int enable) {
   return guarded([&] {
     GZ_REQUIRE(a, "null alg");
     a->a->use_finish = enable != 0;
+  });
+}
+int gz_alg_set_deep_skip(gz_alg* a, int enable) {
+  return guarded([&] {
+    GZ_REQUIRE(a, "null alg");
+    a->a->use_deep_skip = enable != 0;
   });
 }
 uint32_t gz_alg_last_query_finish_round(const gz_alg* a) { return a ? a->a->last_finish_start : 0; }

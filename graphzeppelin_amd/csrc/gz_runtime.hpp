@@ -326,6 +326,7 @@ class GzAlg {
  public:
   bool use_graphs = true;
   bool use_finish = true;           // the single-workgroup finish of the last rounds
+  bool use_deep_skip = true;        // reduces skip the deep records of members whose deep flag is clear
   uint32_t last_finish_start = 0;   // round at which it took over in the last query (0: none)
   static constexpr uint32_t kFirstGraphRounds = 12;
  private:

@@ -191,6 +191,12 @@ size_t gz_alg_last_query_round_sizes(gz_alg* a, uint32_t* entries, uint32_t* gia
  * runs every remaining Borůvka round; 0: every round runs as its own launches.  Results are
  * identical. */
 int gz_alg_set_round_finish(gz_alg* a, int enable);
+/* No counterpart in the reference (a device execution knob).  enable != 0 (the default): a
+ * Borůvka reduce reads a member's records of the round's column that lie past the first line of
+ * the column's overflow run only when the member's deep flag for that column is set (rows with
+ * 4-byte checksums; every write of such a record nonzero sets the flag, and most vertices never
+ * touch those depths); 0: it reads every record.  Results are identical. */
+int gz_alg_set_deep_skip(gz_alg* a, int enable);
 /* No counterpart in the reference (diagnostics).  The Borůvka round from which the last query ran
  * its remaining rounds in the single-workgroup finish, or 0 when it did not. */
 uint32_t gz_alg_last_query_finish_round(const gz_alg* a);

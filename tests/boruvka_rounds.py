@@ -26,8 +26,11 @@ def _good(alpha: np.ndarray, gamma: np.ndarray, seed: int, gamma_bytes: int) -> 
     return (h & mask) == gamma
 
 
-def sample(alpha_sum: np.ndarray, gamma_sum: np.ndarray, col: int, D: int, seed: int, gamma_bytes: int = 4):
-    """Sketch::sample of a merged sketch on column `col` (src/sketch.cpp:94-116): ('Z'|'G'|'F', index)."""
+def sample(alpha_sum: np.ndarray, gamma_sum: np.ndarray, col: int, D: int, seed: int, gamma_bytes: int = 4,
+           depth_out: list | None = None):
+    """Sketch::sample of a merged sketch on column `col` (src/sketch.cpp:94-116): ('Z'|'G'|'F', index).
+
+    depth_out, if given, gets the depth of a good column bucket that was sampled."""
     da, dg = alpha_sum[-1], gamma_sum[-1]
     if da == 0 and dg == 0:
         return 'Z', 0
@@ -36,12 +39,18 @@ def sample(alpha_sum: np.ndarray, gamma_sum: np.ndarray, col: int, D: int, seed:
     ca, cg = alpha_sum[col * D:(col + 1) * D], gamma_sum[col * D:(col + 1) * D]
     good = _good(ca, cg, seed, gamma_bytes)
     if good.any():
-        return 'G', int(ca[int(np.argmax(good))])
+        d = int(np.argmax(good))
+        if depth_out is not None:
+            depth_out.append(d)
+        return 'G', int(ca[d])
     return 'F', 0
 
 
-def partitions(alpha: np.ndarray, gamma: np.ndarray, D: int, seed: int, max_rounds: int, gamma_bytes: int = 4):
-    """Labels (smallest member of each component) after rounds 0, 1, ... until a round changes nothing."""
+def partitions(alpha: np.ndarray, gamma: np.ndarray, D: int, seed: int, max_rounds: int, gamma_bytes: int = 4,
+               depths: list | None = None):
+    """Labels (smallest member of each component) after rounds 0, 1, ... until a round changes nothing.
+
+    depths, if given, gets one list per round: the depth of every good column bucket sampled."""
     n = alpha.shape[0]
     parent = np.arange(n)
 
@@ -58,9 +67,12 @@ def partitions(alpha: np.ndarray, gamma: np.ndarray, D: int, seed: int, max_roun
         bounds = np.flatnonzero(np.diff(roots[order])) + 1
         modified = False
         edges = []
+        round_depths = []
+        if depths is not None:
+            depths.append(round_depths)
         for seg in np.split(order, bounds):
             code, idx = sample(np.bitwise_xor.reduce(alpha[seg], axis=0), np.bitwise_xor.reduce(gamma[seg], axis=0),
-                               r, D, seed, gamma_bytes)
+                               r, D, seed, gamma_bytes, round_depths)
             if code == 'F':
                 modified = True
             elif code == 'G':

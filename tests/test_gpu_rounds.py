@@ -58,6 +58,53 @@ def test_rounds_match_cpu_partitions_erdos(finish):
     assert (took_over > 0) == finish
 
 
+def _complete_bipartite(m):
+    left = np.repeat(np.arange(m, dtype=np.uint32), m)
+    right = np.tile(np.arange(m, 2 * m, dtype=np.uint32), m)
+    return np.stack([left, right], axis=1)
+
+
+@pytest.mark.parametrize("deep_skip", [True, False])
+def test_rounds_match_cpu_partitions_deep_buckets(deep_skip, tmp_path):
+    # K_{1024,1024}: after round 1 the supernodes hold tens to hundreds of vertices of degree 1024,
+    # so their cuts run to 10^4..10^5 edges and their samples come from depths of 14 and more:
+    # records past the first line of a column's overflow run (depth >= 15 with 12-byte records),
+    # which a reduce reads only for members whose deep flag is set (about 3% of them here, at
+    # 1024 updates per vertex).  Every round runs as its own reduce (finish off) and must give the
+    # CPU restatement's partition; so must a reloaded checkpoint (flags set by the overwrite) and
+    # an instance built from raw bucket images (flags set by the XOR merge).
+    n, m, seed = 2048, 1024, 9
+    pairs = _complete_bipartite(m)
+    ref = oracle.CCAlg(n, seed, eager=False)
+    ref.ingest_pairs(pairs)
+    vl = oracle.calc_vector_length(n)
+    D = oracle.calc_bkt_per_col(vl)
+    C = oracle.calc_cc_samples(n, 1.0)
+    alpha, gamma = parse_images(ref.dump(), n, C * D + 1)
+    depths = []
+    partitions(alpha, gamma, D, seed, C, depths=depths)
+    assert max(max(d, default=0) for d in depths[1:]) >= 15, depths
+    cfg = gz.CCAlgConfiguration(eager_dsu=False)
+
+    def check(alg):
+        alg.set_round_finish(False)
+        alg.set_deep_skip(deep_skip)
+        assert _check_query(alg, ref, n, seed) == 0
+
+    alg = gz.CCSketchAlg(n, seed, cfg)
+    alg.ingest_pairs(pairs)
+    check(alg)
+    path = str(tmp_path / "alg.bin")
+    alg.write_binary(path)
+    check(gz.CCSketchAlg.construct_from_serialized_data(path, cfg))
+    raw = gz.CCSketchAlg(n, seed, cfg)
+    images = ref.dump().reshape(n, -1)
+    for v in range(n):
+        raw.apply_raw_buckets_update(v, images[v].tobytes())
+    raw.pre_insert(0, m)  # without the eager DSU this invalidates the cached (empty) answer
+    check(raw)
+
+
 def test_rounds_match_cpu_partitions_power_law():
     # a heavy-tailed graph: hubs make large supernodes early, so candidate giants (a supernode with
     # at least n/32 vertices, excluded a round before it holds a majority) are exercised
