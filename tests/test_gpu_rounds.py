@@ -20,12 +20,12 @@ from boruvka_rounds import parse_images, partitions
 pytestmark = pytest.mark.gpu
 
 
-def _check_query(alg, ref, n, seed):
+def _check_query(alg, ref, n, seed, gb=4):
     vl = oracle.calc_vector_length(n)
     D = oracle.calc_bkt_per_col(vl)
     C = oracle.calc_cc_samples(n, 1.0)
-    alpha, gamma = parse_images(ref.dump(), n, C * D + 1)
-    parts = partitions(alpha, gamma, D, seed, C)
+    alpha, gamma = parse_images(ref.dump(), n, C * D + 1, gb)
+    parts = partitions(alpha, gamma, D, seed, C, gb)
     cc = alg.connected_components()
     rounds = alg.last_query_rounds
     assert rounds == len(parts) - 1
@@ -64,32 +64,35 @@ def _complete_bipartite(m):
     return np.stack([left, right], axis=1)
 
 
-@pytest.mark.parametrize("deep_skip", [True, False])
-def test_rounds_match_cpu_partitions_deep_buckets(deep_skip, tmp_path):
+@pytest.mark.parametrize("gb,deep_skip", [(4, True), (4, False), (8, True)])
+def test_rounds_match_cpu_partitions_deep_buckets(gb, deep_skip, tmp_path):
     # K_{1024,1024}: after round 1 the supernodes hold tens to hundreds of vertices of degree 1024,
     # so their cuts run to 10^4..10^5 edges and their samples come from depths of 14 and more:
     # records past the first line of a column's overflow run (depth >= 15 with 12-byte records),
     # which a reduce reads only for members whose deep flag is set (about 3% of them here, at
     # 1024 updates per vertex).  Every round runs as its own reduce (finish off) and must give the
     # CPU restatement's partition; so must a reloaded checkpoint (flags set by the overwrite) and
-    # an instance built from raw bucket images (flags set by the XOR merge).
+    # an instance built from raw bucket images (flags set by the XOR merge).  The sketch images
+    # stay bit-exact: the flags live outside the records.  Rows of 16-byte records keep no flags
+    # (byte 60 of a half-line is inside a record there) and read every record.
     n, m, seed = 2048, 1024, 9
     pairs = _complete_bipartite(m)
-    ref = oracle.CCAlg(n, seed, eager=False)
+    ref = oracle.CCAlg(n, seed, gamma_bytes=gb, eager=False)
     ref.ingest_pairs(pairs)
     vl = oracle.calc_vector_length(n)
     D = oracle.calc_bkt_per_col(vl)
     C = oracle.calc_cc_samples(n, 1.0)
-    alpha, gamma = parse_images(ref.dump(), n, C * D + 1)
+    alpha, gamma = parse_images(ref.dump(), n, C * D + 1, gb)
     depths = []
-    partitions(alpha, gamma, D, seed, C, depths=depths)
+    partitions(alpha, gamma, D, seed, C, gb, depths=depths)
     assert max(max(d, default=0) for d in depths[1:]) >= 15, depths
-    cfg = gz.CCAlgConfiguration(eager_dsu=False)
+    cfg = gz.CCAlgConfiguration(eager_dsu=False, gamma_bytes=gb)
 
     def check(alg):
         alg.set_round_finish(False)
         alg.set_deep_skip(deep_skip)
-        assert _check_query(alg, ref, n, seed) == 0
+        assert np.array_equal(alg.dump_sketches(), ref.dump())
+        assert _check_query(alg, ref, n, seed, gb) == 0
 
     alg = gz.CCSketchAlg(n, seed, cfg)
     alg.ingest_pairs(pairs)
