@@ -85,6 +85,34 @@ def test_config1_sketches_bit_exact(gb, path):
 
 
 @pytest.mark.parametrize("gb", [4, 8])
+def test_wide_sketches_bit_exact(gb):
+    # more than 28 columns: the update kernel's write-back takes more than one pass over its LDS
+    # delta (a separate kernel instance); images, partition and round count as the oracle's
+    say(f"wide sketches gb={gb}")
+    n, seed, sf = 300, 5, 3.0
+    assert oracle.calc_cc_samples(n, sf) == 34
+    pairs, _ = streams.erdos_renyi_churn_stream(n, 0.05, 11)
+    ref = oracle.CCAlg(n, seed, sketches_factor=sf, gamma_bytes=gb, eager=False)
+    ref.ingest_pairs(pairs)
+    alg = gz.CCSketchAlg(n, seed, gz.CCAlgConfiguration(sketches_factor=sf, gamma_bytes=gb, eager_dsu=False))
+    alg.ingest_pairs(pairs)
+    assert np.array_equal(alg.dump_sketches(), ref.dump())
+    cc = alg.connected_components()
+    rpar, rc = ref.connected_components()
+    c, lab = oracle.exact_cc(n, pairs)
+    assert cc.size() == rc == c and np.array_equal(cc.canonical_labels(), lab)
+    assert np.array_equal(cc.canonical_labels(), oracle.canonical(rpar))
+    assert alg.last_query_rounds == ref.last_query_rounds
+    # a standalone sketch of 35 columns, one vertex split over several tiles (atomic write-back)
+    idx = np.random.default_rng(gb).integers(0, 1 << 30, size=3000, dtype=np.uint64)
+    g = gz.Sketch(1 << 30, 77, 5, 7, gb)
+    r = oracle.Sketch(1 << 30, 77, 5, 7, gb)
+    g.update_batch(idx)
+    r.update_batch(idx)
+    assert g.serialize() == r.serialize()
+
+
+@pytest.mark.parametrize("gb", [4, 8])
 def test_standalone_sketch_parity(gb):
     say(f"standalone sketch gb={gb}")
     rng = np.random.default_rng(gb)
